@@ -1,0 +1,220 @@
+/*
+ * ppc_hip.h -- C-ABI of the MI355X photon-propagation library (libppc_hip.so).
+ *
+ * This is the seam the reference's CUDA backend fills in gpu/ppc.cu: the host
+ * side (tables, front-ends, hit post-processing) hands packed tables and photon
+ * bunches to the device and collects raw hits.  Plain C types only; every
+ * function returns 0 on success or a negative ppc_hip_status code, with a
+ * message available from ppc_hip_last_error().
+ *
+ * One context drives one GPU.  A context is not thread safe; calls on one
+ * context must come from one host thread (as in the reference, gpu/ppc.cu).
+ */
+#ifndef PPC_HIP_H
+#define PPC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ppc_hip_status {
+  PPC_HIP_OK = 0,
+  PPC_HIP_E_ARG = -1,      /* invalid argument */
+  PPC_HIP_E_DEVICE = -2,   /* HIP runtime error (the reference exits with code 2) */
+  PPC_HIP_E_NOMEM = -3,    /* device memory exhausted */
+  PPC_HIP_E_STATE = -4,    /* call out of order (e.g. launch before upload) */
+  PPC_HIP_E_CAPACITY = -5  /* a size exceeds the configured capacity */
+};
+
+/* photon bunch: the reference's `photon` (gpu/ini.cxx:399-419), 64 bytes */
+typedef struct {
+  float r[4];     /* start position [m], time [ns] */
+  float n[4];     /* direction; n[3] = track length (particles), 0/1 for flashers */
+  uint32_t q;     /* track segment number, copied to hits */
+  uint32_t num;   /* number of photons in this bunch */
+  int32_t type;   /* >0: flasher/LED (1..4); <=0: particle (-type of the light yield model) */
+  float f;        /* fraction of light from the bare muon */
+  union {
+    struct {
+      float a, b;   /* longitudinal profile: offset = b * Gamma(a) */
+      float beta;   /* particle velocity */
+      float tau;    /* luminescence decay time */
+    } p;
+    struct {
+      float ka, up; /* beam width parameter, LED elevation [rad] */
+      float fldr;   /* LED #1 azimuth [deg] (<0: azimuthally symmetric) */
+      int16_t fla, ofla; /* flashing DOM index, DOM excluded from hits */
+    } l;
+  } u;
+} ppc_hip_bunch;
+
+/* raw hit: the reference's `hit` (gpu/ini.cxx:367-371), 32 bytes */
+typedef struct {
+  uint32_t i;     /* DOM index */
+  uint32_t n;     /* track segment number (bunch q) */
+  uint32_t z;     /* wavelength bin */
+  float t;        /* arrival time [ns] */
+  float pth, pph; /* photon direction at arrival (zenith, azimuth) */
+  float dth, dph; /* impact point on the DOM (zenith, azimuth) */
+} ppc_hip_hit;
+
+/* string (line) of the cell grid: the reference's `line` (gpu/ini.cxx:438-443) */
+typedef struct {
+  int32_t n, max; /* first DOM index, number of DOMs - 1 */
+  float x, y, r;  /* centre and search radius */
+  float h, d;     /* highest DOM z, 1/spacing */
+  float dl, dh;   /* low/high z spread around the regular spacing */
+} ppc_hip_line;
+
+/* DOM: the reference's `DOM` (gpu/ini.cxx:42-45) */
+typedef struct {
+  float R, F;  /* radius, vertical/horizontal axis ratio (<0: cylinder) */
+  float r[3];  /* position [m] */
+} ppc_hip_dom;
+
+#define PPC_HIP_WNUM 512
+#define PPC_HIP_MAXLYS 172
+#define PPC_HIP_LMAX 80
+#define PPC_HIP_LYRS 172
+#define PPC_HIP_NSTR 240
+#define PPC_HIP_CX 21
+#define PPC_HIP_CY 19
+#define PPC_HIP_CTX 11
+#define PPC_HIP_CTY 10
+
+/*
+ * Tables built by the host from the ice directory (what gpu/ini.cxx:578-1700
+ * puts into `dats`/`datz`), with the per-wavelength ice tables factorised:
+ *   sca(bin, layer) = lay[layer].s0 + lay[layer].s1 * wv[bin].la
+ *   abs(bin, layer) = lay[layer].p  * wv[bin].lk + lay[layer].t * wv[bin].abl
+ * which restates gpu/ini.cxx:1677-1689 term by term.
+ */
+typedef struct {
+  float s0, s1, p, t;
+} ppc_hip_layer;
+
+typedef struct {
+  float k1, k2, ra, rb; /* gpu/ini.cxx:431-436 */
+} ppc_hip_aniz;
+
+typedef struct {
+  float ocm;            /* group index / c [ns/m] */
+  float coschr, sinchr; /* Cherenkov angle */
+  float la, lk, abl;    /* wavelength factors of the factorised ice tables */
+  int32_t wvl;          /* bin number written into hits */
+  float pad;
+} ppc_hip_wv;
+
+typedef struct {
+  /* layers (gpu/ini.cxx:1543-1562) */
+  int32_t size;
+  float dh, rdh, hmin;
+  /* light / scattering / DOMs (gpu/ini.cxx:646-691) */
+  float ocv, sf, g, gr, xR;
+  float SF, G, GR, hr, hr2, hs, ha, hifl;
+  float azx, azy, k1, k2, kz, fr;
+  float sum, bfr[12];
+  /* cell grid (gpu/ini.cxx:1083-1174) */
+  float rx;
+  int32_t cn[2];
+  float cl[2], crst[2], cb[4]; /* cb[i*2+j] */
+  /* tilt (gpu/ini.cxx:1178-1319) */
+  int32_t tmod, vthk, lnum, lpts;
+  float lmin, lrdz, r0, lnx, lny;
+  float lr[PPC_HIP_LMAX];
+  float mmin[2], mstp[2];
+  int32_t mnum[2], mcut[2];
+  int32_t mcol[PPC_HIP_CTX * PPC_HIP_CTY];
+  /* arrays (host pointers; copied at upload) */
+  const ppc_hip_layer* lay;  /* [size] */
+  const ppc_hip_aniz* az;    /* [size] */
+  const ppc_hip_wv* wv;      /* [PPC_HIP_WNUM] */
+  const uint16_t* is;        /* [PPC_HIP_CX*PPC_HIP_CY], is[i*CY+j] */
+  const uint16_t* ls;        /* [PPC_HIP_NSTR] */
+  int32_t nsc;
+  const ppc_hip_line* sc;    /* [nsc] */
+  const float* lp;           /* [lnum][PPC_HIP_LYRS][2] tilt table (value, slope) */
+  int32_t gsize;
+  const ppc_hip_dom* oms;    /* [gsize] */
+  int32_t rsize;
+  const uint32_t* rm;        /* [rsize] MWC multipliers, one per concurrent lane */
+} ppc_hip_tables;
+
+/* launch shape and capacities (the reference derives these from the device,
+ * gpu/ppc.cu:141-186; 0 selects the library default) */
+typedef struct {
+  int32_t blocks;          /* workgroups; default: 2 per CU */
+  int32_t threads;         /* lanes per workgroup (multiple of 64); default 256 */
+  int32_t refill;          /* refill a wave once this many of its 64 lanes are idle; default 16 */
+  int32_t deterministic;   /* 1: photon i always runs on lane i % lanes (the reference's DTMN) */
+  uint32_t max_bunches;    /* bunch buffer capacity; default 1<<20 */
+  uint32_t max_hits;       /* hit buffer capacity; default 1<<24 */
+  uint32_t rec_stride;     /* record the MWC state at emission of every rec_stride-th photon (0: off) */
+  uint32_t rec_max;        /* capacity of the record buffer */
+} ppc_hip_config;
+
+/* per-launch results (what gpu/ppc.cu:255-283 reads back) */
+typedef struct {
+  uint64_t photons;   /* photons propagated */
+  uint64_t steps;     /* transport steps (loop iterations) */
+  uint64_t hits;      /* raw hits produced (may exceed max_hits: overflow) */
+  uint64_t stored;    /* hits stored in the buffer */
+  uint64_t killed;    /* photons killed at emission (ofla < -1) */
+  uint64_t nonfinite; /* photons dropped because TOT/SCA became inf/nan */
+  float device_ms;    /* kernel time between HIP events on the launch stream */
+  int32_t lanes;      /* lanes of the launch */
+} ppc_hip_stats;
+
+/* replay record: MWC state of a lane when it starts emitting photon `photon` */
+typedef struct {
+  uint32_t photon;
+  uint32_t x, c, a;
+} ppc_hip_record;
+
+typedef struct ppc_hip_ctx ppc_hip_ctx;
+
+/* number of visible devices (listDevices, gpu/ppc.cu:404-425) */
+int ppc_hip_device_count(int32_t* count);
+/* name, CU count and memory of a device */
+int ppc_hip_device_info(int32_t device, char* name, size_t name_len, int32_t* cus, uint64_t* mem);
+
+/* choose(device) + gpu::gpu (gpu/ppc.cu:117-149, 490-508) */
+int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** ctx);
+/* free all device memory (gpu::fin + stop, gpu/ppc.cu:239-249,308-311) */
+int ppc_hip_close(ppc_hip_ctx* ctx);
+
+/* lanes of a launch; the tables must carry at least this many multipliers */
+int ppc_hip_lanes(const ppc_hip_ctx* ctx, int32_t* lanes);
+/* maximum bunches and hits per launch (getMaxBunchSize, gpu/ppc.cu:368) */
+int ppc_hip_capacity(const ppc_hip_ctx* ctx, uint32_t* max_bunches, uint32_t* max_hits);
+
+/* copy tables to the device (gpu::ini, gpu/ppc.cu:151-237) */
+int ppc_hip_upload(ppc_hip_ctx* ctx, const ppc_hip_tables* tables);
+/* reset every lane's MWC state to x = 362436069 + sv, c = 1234567 (rs_ini, gpu/ini.cxx:556-571) */
+int ppc_hip_seed(ppc_hip_ctx* ctx, uint32_t sv);
+/* read/write the per-lane MWC states (x | c << 32), lanes entries */
+int ppc_hip_get_states(ppc_hip_ctx* ctx, uint64_t* states, int32_t lanes);
+int ppc_hip_set_states(ppc_hip_ctx* ctx, const uint64_t* states, int32_t lanes);
+
+/* start propagating num photons of nb bunches (kernel_c + kernel_f,
+ * gpu/ppc.cu:285-306); returns without waiting for the kernel */
+int ppc_hip_launch(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num);
+/* wait for the launch and read its counters (kernel_i, gpu/ppc.cu:255-277) */
+int ppc_hip_wait(ppc_hip_ctx* ctx, ppc_hip_stats* stats);
+/* copy min(stored, cap) hits of the last launch to the host (gpu/ppc.cu:278-282);
+ * photon (optional) receives the photon index of each hit */
+int ppc_hip_hits(ppc_hip_ctx* ctx, ppc_hip_hit* hits, uint32_t* photon, uint32_t cap, uint32_t* n);
+/* the replay records of the last launch */
+int ppc_hip_records(ppc_hip_ctx* ctx, ppc_hip_record* rec, uint32_t cap, uint32_t* n);
+
+/* message of the last failing call (thread local) */
+const char* ppc_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,68 @@
+/*
+ * xppc_c.h -- C facade of libxppc.so for foreign-function callers.
+ *
+ * The reference exposes its front-end only as the C++ namespace xppc
+ * (llh/ppc.h); this facade gives the same operations plain C signatures so that
+ * ctypes/cgo/JNI-style bindings can build the tables from an ice directory,
+ * make the photon bunches the front-ends make, and hand both to the device
+ * library (ppc_hip.h).  Every function refers to the single process-wide
+ * detector that xppc::initialize() builds.
+ */
+#ifndef XPPC_C_H
+#define XPPC_C_H
+
+#include <stdint.h>
+
+#include "ppc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* the first n multipliers a < `below` with a*2^32-1 and (a*2^32-2)/2 prime, in
+ * descending order (the definition of the ice directories' rnd.txt); returns n */
+int ppc_mwc_multipliers(uint32_t below, uint32_t n, uint32_t* out);
+
+/* xppc::initialize (gpu/ppc.cu:36): read the ice directory named by the
+ * PPCTABLESDIR / ICEMODELDIR / ... environment (gpu/ini.cxx:584-625) */
+int xppc_c_initialize(float enh);
+
+/* the device tables of the initialised detector */
+const ppc_hip_tables* xppc_c_tables(void);
+
+/* pointer to and size of a named table of the initialised detector:
+ * scalars and arrays of ppc_hip_tables by member name, plus
+ *   abs_, sca     dense [512][172] ice tables as gpu/ini.cxx:1684-1691 builds them
+ *   str, dom      string and DOM number of every in-ice DOM (int32)
+ *   wvs           [512][3] wavelength bin centre, low and high edge [nm]
+ *   eff, ovr, l0  light-yield scale, oversize factor xR^2, zero string of the 1-d tilt */
+const void* xppc_c_field(const char* name, uint64_t* bytes);
+
+/* lengthen the multiplier table to at least `lanes` entries, continuing the
+ * safe-prime sequence below the smallest multiplier present; returns rsize */
+int xppc_c_extend_multipliers(int32_t lanes);
+
+/* the flasher bunch flset(str, dom) configures (gpu/f2k.cxx:630-759);
+ * num is left at 0 */
+int xppc_c_flasher_bunch(int32_t str, int32_t dom, ppc_hip_bunch* out);
+
+/* the bunch addp() builds for a particle of f2k type (0 muon, 11-14 EM,
+ * 101-107 hadron; gpu/f2k.cxx:17-167,484-504) with direction n; returns the
+ * Poisson mean of its photon count, num is left at 0 */
+double xppc_c_particle_bunch(float x, float y, float z, float t, float E, float dr, int32_t type, float nx,
+                             float ny, float nz, ppc_hip_bunch* out);
+
+/* photons flone(num) propagates for a flasher pulse: num*eff/dppm/xR^2 (gpu/f2k.cxx:586-588) */
+uint64_t xppc_c_flasher_photons(uint64_t num);
+
+/* depth shift of the ice layers at (x,y,z) and the local layer thickness (gpu/pro.cu:168-264) */
+float xppc_c_zshift(float x, float y, float z, float* dz);
+
+/* the host random-number state (seed of the MWC streams, gpu/ini.cxx:554) */
+uint32_t xppc_c_seed(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,154 @@
+// ref_harness.cpp -- C entry points around the reference's own CPU build.
+//
+// TEST INFRASTRUCTURE ONLY (see ppc_oracle.h).  This translation unit compiles
+// the reference's gpu/ppc.cu from where it lies (the include path is given by
+// oracle/Makefile) in its single-thread CPU form with -DXLIB -DDTMN, and adds a
+// few extern "C" functions so that tests can
+//   * export the tables ini::set() built (gpu/ini.cxx:578-1700) into orc_tables,
+//   * run the reference kernel (gpu/pro.cu:287-904) on given bunches from a
+//     given MWC state, in the single-thread order of gpu/ppc.cu:439-440, and
+//     collect its raw hits.
+// Nothing of the reference is copied: it is #included unchanged.
+#include "ppc.cu"
+
+#include "ppc_oracle.h"
+
+using namespace xppc;
+
+extern "C" {
+
+int ref_initialize(void) {
+  initialize(1.f);
+  return 0;
+}
+
+// choose(dev) + ini(): one stream (seed = dev), buffers for 512*1024 photons
+void ref_start(int dev) {
+  choose(dev);
+  ini();
+}
+
+uint32_t ref_pmax(void) { return pmax; }
+
+float ref_zshift(float x, float y, float zz, float* dz) {
+  float4 r;
+  r.x = x, r.y = y, r.z = zz, r.w = 0;
+  float h = d.dh;
+  float s = zshift(d, r, h);
+  if (dz) *dz = h;
+  return s;
+}
+
+void ref_export(orc_tables* t) {
+  t->size = d.size;
+  t->dh = d.dh, t->rdh = d.rdh, t->hmin = d.hmin;
+  t->ocv = d.ocv, t->sf = d.sf, t->g = d.g, t->gr = d.gr, t->xR = d.xR;
+  t->SF = d.SF, t->G = d.G, t->GR = d.GR;
+  t->hr = d.hr, t->hr2 = d.hr2, t->hs = d.hs, t->ha = d.ha, t->hifl = d.hifl;
+  t->azx = d.azx, t->azy = d.azy, t->k1 = d.k1, t->k2 = d.k2, t->kz = d.kz, t->fr = d.fr;
+  t->sum = d.sum;
+  for (int i = 0; i < 12; i++) t->bfr[i] = d.bfr[i];
+  for (int i = 0; i < MAXLYS; i++) {
+    t->az_k1[i] = d.az[i].k1, t->az_k2[i] = d.az[i].k2;
+    t->az_ra[i] = d.az[i].ra, t->az_rb[i] = d.az[i].rb;
+  }
+  t->rx = d.rx;
+  for (int m = 0; m < 2; m++) t->cn[m] = d.cn[m], t->cl[m] = d.cl[m], t->crst[m] = d.crst[m];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) t->cb[i * 2 + j] = d.cb[i][j];
+  for (int i = 0; i < NSTR; i++) t->ls[i] = d.ls[i];
+  for (int i = 0; i < CX; i++)
+    for (int j = 0; j < CY; j++) t->is_[i * CY + j] = d.is[i][j];
+  // number of string lines: count entries referenced by the cell lists
+  int nsc = 0;
+  for (int i = 0; i < NSTR; i++) {
+    const line& s = d.sc[i];
+    t->sc[i].n = s.n, t->sc[i].max = s.max;
+    t->sc[i].x = s.x, t->sc[i].y = s.y, t->sc[i].r = s.r;
+    t->sc[i].h = s.h, t->sc[i].d = s.d, t->sc[i].dl = s.dl, t->sc[i].dh = s.dh;
+  }
+  for (int i = 0; i < d.cn[0]; i++)
+    for (int j = 0; j < d.cn[1]; j++)
+      for (unsigned short k = d.is[i][j]; k != 0x8000;) {
+        unsigned short m = d.ls[k];
+        if ((m & 0x7fff) + 1 > nsc) nsc = (m & 0x7fff) + 1;
+        k = m & 0x8000 ? 0x8000 : k + 1;
+      }
+  t->nsc = nsc;
+  t->tmod = d.tmod, t->vthk = d.vthk, t->lnum = d.lnum, t->lpts = d.lpts, t->l0 = d.l0;
+  t->lmin = d.lmin, t->lrdz = d.lrdz, t->r0 = d.r0, t->lnx = d.lnx, t->lny = d.lny;
+  for (int i = 0; i < LMAX; i++) t->lr[i] = d.lr[i];
+  for (int m = 0; m < 2; m++) {
+    t->mmin[m] = d.mmin[m], t->mstp[m] = d.mstp[m], t->mnum[m] = d.mnum[m], t->mcut[m] = d.mcut[m];
+  }
+  for (int i = 0; i < CTX; i++)
+    for (int j = 0; j < CTY; j++) t->mcol[i * CTY + j] = d.mcol[i][j];
+  for (int i = 0; i < LMAX; i++)
+    for (int j = 0; j < LYRS; j++) {
+      t->lp_x[i * LYRS + j] = z.lp[i][j].x;
+      t->lp_y[i * LYRS + j] = z.lp[i][j].y;
+    }
+  for (int n = 0; n < WNUM; n++) {
+    const ices& w = z.w[n];
+    t->wvl[n] = w.wvl, t->ocm[n] = w.ocm, t->coschr[n] = w.coschr, t->sinchr[n] = w.sinchr;
+    for (int i = 0; i < MAXLYS; i++) {
+      t->abs_[n * MAXLYS + i] = w.z[i].abs;
+      t->sca[n * MAXLYS + i] = w.z[i].sca;
+    }
+    t->wvs_w[n] = q.wvs[n].w, t->wvs_i[n] = q.wvs[n].i, t->wvs_f[n] = q.wvs[n].f;
+  }
+  t->eff = q.eff;
+  t->gsize = d.gsize;
+  for (int i = 0; i < d.gsize; i++) {
+    const DOM& o = q.oms[i];
+    t->oms[i].R = o.R, t->oms[i].F = o.F;
+    t->oms[i].x = o.r[0], t->oms[i].y = o.r[1], t->oms[i].z = o.r[2];
+    t->str[i] = q.names[i].str, t->dom[i] = q.names[i].dom;
+  }
+  t->rsize = d.rsize;
+  t->l0 = d.l0;
+}
+
+uint32_t ref_multiplier(uint32_t k) { return z.rm[k]; }
+
+static void to_photon(const orc_bunch& b, photon& p) {
+  p.r.x = b.r[0], p.r.y = b.r[1], p.r.z = b.r[2], p.r.w = b.r[3];
+  p.n.x = b.n[0], p.n.y = b.n[1], p.n.z = b.n[2], p.n.w = b.n[3];
+  p.q = b.q, p.num = b.num, p.type = b.type, p.f = b.f;
+  if (b.type > 0) {
+    p.ka = b.ka, p.up = b.up, p.fldr = b.fldr;
+    p.fla = (short)b.fla, p.ofla = (short)b.ofla;
+  } else {
+    p.a = b.a, p.b = b.b, p.beta = b.beta, p.tau = b.tau;
+  }
+}
+
+// Run num photons of the given bunches through the reference kernel with nthr
+// single-thread "threads" sharing MWC stream `stream` (multiplier mult, state
+// x | c<<32).  Returns the number of raw hits (copied up to cap).
+uint32_t ref_run(const orc_bunch* bunches, int nb, uint32_t num, uint32_t nthr, uint32_t stream,
+                 uint32_t mult, uint32_t sx, uint32_t sc, orc_hit* hits, uint32_t cap, uint32_t* out_x,
+                 uint32_t* out_c) {
+  if (num > pmax || (unsigned)nb > pmxo || nthr < 1) return 0xffffffffu;
+  seed = stream;
+  z.rm[stream % d.rsize] = mult;
+  z.rs[stream % d.rsize] = sx | (unsigned long long)sc << 32;
+  for (int k = 0; k < nb; k++) to_photon(bunches[k], q.pz[k]);
+  d.hidx = 0;
+  d.blockIdx = 0, d.gridDim = 1, blockDim.x = nthr;
+  for (threadIdx.x = 0; threadIdx.x < (int)nthr; threadIdx.x++) propagate(e, num);
+  unsigned long long st = z.rs[stream % d.rsize];
+  if (out_x) *out_x = (uint32_t)st;
+  if (out_c) *out_c = (uint32_t)(st >> 32);
+  uint32_t nh = d.hidx;
+  uint32_t m = nh < d.hnum ? nh : d.hnum;
+  if (m > cap) m = cap;
+  for (uint32_t k = 0; k < m; k++) {
+    const hit& h = q.hits[k];
+    hits[k].i = h.i, hits[k].n = h.n, hits[k].z = h.z, hits[k].t = h.t;
+    hits[k].pth = h.pth, hits[k].pph = h.pph, hits[k].dth = h.dth, hits[k].dph = h.dph;
+  }
+  return nh;
+}
+
+}  // extern "C"

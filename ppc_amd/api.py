@@ -1,0 +1,273 @@
+"""Python binding of the C-ABI (include/ppc_hip.h) and of the xppc C facade
+(include/xppc_c.h).
+
+``Detector`` builds the tables from an ice directory (what ``xppc::initialize``
+does, gpu/ppc.cu:36) and makes photon bunches the way the reference front-ends
+do.  ``Device`` owns one GPU context of libppc_hip.so: upload, launch, wait,
+hits, replay records.  There is no fallback: without the HIP library or a GPU
+the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _native
+
+PPC_HIP_WNUM = 512
+
+
+class _LP(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_float), ("b", ctypes.c_float), ("beta", ctypes.c_float), ("tau", ctypes.c_float)]
+
+
+class _LL(ctypes.Structure):
+    _fields_ = [("ka", ctypes.c_float), ("up", ctypes.c_float), ("fldr", ctypes.c_float),
+                ("fla", ctypes.c_int16), ("ofla", ctypes.c_int16)]
+
+
+class _LU(ctypes.Union):
+    _fields_ = [("p", _LP), ("l", _LL)]
+
+
+class Bunch(ctypes.Structure):
+    """ppc_hip_bunch: the reference's photon bunch (gpu/ini.cxx:399-419)."""
+    _fields_ = [("r", ctypes.c_float * 4), ("n", ctypes.c_float * 4), ("q", ctypes.c_uint32),
+                ("num", ctypes.c_uint32), ("type", ctypes.c_int32), ("f", ctypes.c_float), ("u", _LU)]
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("i", ctypes.c_uint32), ("n", ctypes.c_uint32), ("z", ctypes.c_uint32), ("t", ctypes.c_float),
+                ("pth", ctypes.c_float), ("pph", ctypes.c_float), ("dth", ctypes.c_float), ("dph", ctypes.c_float)]
+
+
+HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"), ("pph", "<f4"),
+                      ("dth", "<f4"), ("dph", "<f4")])
+RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<u4")])
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32), ("refill", ctypes.c_int32),
+                ("deterministic", ctypes.c_int32), ("max_bunches", ctypes.c_uint32), ("max_hits", ctypes.c_uint32),
+                ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("photons", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("hits", ctypes.c_uint64),
+                ("stored", ctypes.c_uint64), ("killed", ctypes.c_uint64), ("nonfinite", ctypes.c_uint64),
+                ("device_ms", ctypes.c_float), ("lanes", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PPCError(RuntimeError):
+    pass
+
+
+_hip = None
+_xppc = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        lib = _native.hip_lib()
+        c_p = ctypes.c_void_p
+        lib.ppc_hip_last_error.restype = ctypes.c_char_p
+        lib.ppc_hip_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+        lib.ppc_hip_device_info.argtypes = [ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint64)]
+        lib.ppc_hip_open.argtypes = [ctypes.c_int32, ctypes.POINTER(Config), ctypes.POINTER(c_p)]
+        lib.ppc_hip_close.argtypes = [c_p]
+        lib.ppc_hip_lanes.argtypes = [c_p, ctypes.POINTER(ctypes.c_int32)]
+        lib.ppc_hip_capacity.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_upload.argtypes = [c_p, c_p]
+        lib.ppc_hip_seed.argtypes = [c_p, ctypes.c_uint32]
+        lib.ppc_hip_get_states.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]
+        lib.ppc_hip_set_states.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]
+        lib.ppc_hip_launch.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32]
+        lib.ppc_hip_wait.argtypes = [c_p, ctypes.POINTER(Stats)]
+        lib.ppc_hip_hits.argtypes = [c_p, c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_records.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        _hip = lib
+    return _hip
+
+
+def xppc():
+    global _xppc
+    if _xppc is None:
+        hip()  # libxppc links libppc_hip
+        lib = _native.xppc_lib()
+        lib.xppc_c_initialize.argtypes = [ctypes.c_float]
+        lib.xppc_c_tables.restype = ctypes.c_void_p
+        lib.xppc_c_field.restype = ctypes.c_void_p
+        lib.xppc_c_field.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+        lib.xppc_c_extend_multipliers.argtypes = [ctypes.c_int32]
+        lib.xppc_c_flasher_bunch.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Bunch)]
+        lib.xppc_c_particle_bunch.restype = ctypes.c_double
+        lib.xppc_c_particle_bunch.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int32] + [ctypes.c_float] * 3 + [
+            ctypes.POINTER(Bunch)]
+        lib.xppc_c_flasher_photons.restype = ctypes.c_uint64
+        lib.xppc_c_flasher_photons.argtypes = [ctypes.c_uint64]
+        lib.xppc_c_zshift.restype = ctypes.c_float
+        lib.xppc_c_zshift.argtypes = [ctypes.c_float] * 3 + [ctypes.POINTER(ctypes.c_float)]
+        lib.xppc_c_seed.restype = ctypes.c_uint32
+        lib.ppc_mwc_multipliers.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        _xppc = lib
+    return _xppc
+
+
+def _check(rc):
+    if rc != 0:
+        raise PPCError("ppc_hip error %d: %s" % (rc, hip().ppc_hip_last_error().decode()))
+
+
+class Detector:
+    """The process-wide detector of libxppc (one per process, as in the reference)."""
+
+    def __init__(self, icedir, env=None, enh=1.0):
+        os.environ["PPCTABLESDIR"] = icedir
+        for k, v in (env or {}).items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = str(v)
+        self.lib = xppc()
+        self.lib.xppc_c_initialize(ctypes.c_float(enh))
+
+    def field(self, name):
+        nb = ctypes.c_uint64()
+        p = self.lib.xppc_c_field(name.encode(), ctypes.byref(nb))
+        if not p:
+            raise KeyError(name)
+        return ctypes.string_at(p, nb.value)
+
+    def array(self, name, dtype):
+        return np.frombuffer(self.field(name), dtype=dtype).copy()
+
+    def scalar(self, name, dtype=np.float32):
+        return self.array(name, dtype)[0]
+
+    @property
+    def tables(self):
+        return self.lib.xppc_c_tables()
+
+    def extend_multipliers(self, lanes):
+        n = self.lib.xppc_c_extend_multipliers(lanes)
+        if n < lanes:
+            raise PPCError("could not extend the multiplier table to %d" % lanes)
+        return n
+
+    def flasher_bunch(self, string, dom, num=0, seg=0):
+        b = Bunch()
+        if self.lib.xppc_c_flasher_bunch(string, dom, ctypes.byref(b)) != 0:
+            raise PPCError("flasher configuration failed")
+        b.num, b.q = num, seg
+        return b
+
+    def particle_bunch(self, x, y, z, t, E, dr, ptype, n, num=None, seg=0):
+        """bunch of an f2k particle (type 0 muon, 11-14 EM, 101-107 hadron); num
+        defaults to the mean photon count rounded."""
+        b = Bunch()
+        mean = self.lib.xppc_c_particle_bunch(x, y, z, t, E, dr, ptype, n[0], n[1], n[2], ctypes.byref(b))
+        b.num = int(round(mean)) if num is None else num
+        b.q = seg
+        return b, mean
+
+    def flasher_photons(self, num):
+        return int(self.lib.xppc_c_flasher_photons(num))
+
+    def zshift(self, x, y, z):
+        dz = ctypes.c_float()
+        s = self.lib.xppc_c_zshift(x, y, z, ctypes.byref(dz))
+        return s, dz.value
+
+
+def device_count():
+    n = ctypes.c_int32()
+    _check(hip().ppc_hip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Device:
+    """One GPU context of libppc_hip.so."""
+
+    def __init__(self, device=0, blocks=0, threads=0, refill=0, deterministic=False, max_bunches=0, max_hits=0,
+                 rec_stride=0, rec_max=0):
+        self.lib = hip()
+        self.cfg = Config(blocks, threads, refill, 1 if deterministic else 0, max_bunches, max_hits, rec_stride,
+                          rec_max)
+        self.ctx = ctypes.c_void_p()
+        _check(self.lib.ppc_hip_open(device, ctypes.byref(self.cfg), ctypes.byref(self.ctx)))
+        n = ctypes.c_int32()
+        _check(self.lib.ppc_hip_lanes(self.ctx, ctypes.byref(n)))
+        self.lanes = n.value
+        b, h = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self.lib.ppc_hip_capacity(self.ctx, ctypes.byref(b), ctypes.byref(h)))
+        self.max_bunches, self.max_hits = b.value, h.value
+
+    def close(self):
+        if self.ctx:
+            self.lib.ppc_hip_close(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, det):
+        det.extend_multipliers(self.lanes)
+        _check(self.lib.ppc_hip_upload(self.ctx, det.tables))
+
+    def seed(self, sv):
+        _check(self.lib.ppc_hip_seed(self.ctx, sv))
+
+    def states(self):
+        a = np.zeros(self.lanes, dtype=np.uint64)
+        _check(self.lib.ppc_hip_get_states(self.ctx, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), self.lanes))
+        return a
+
+    def set_states(self, a):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        _check(self.lib.ppc_hip_set_states(self.ctx, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), self.lanes))
+
+    def launch(self, bunches, num=None):
+        if isinstance(bunches, Bunch):
+            bunches = [bunches]
+        arr = (Bunch * len(bunches))(*bunches)
+        if num is None:
+            num = sum(b.num for b in bunches)
+        _check(self.lib.ppc_hip_launch(self.ctx, arr, len(bunches), num))
+
+    def wait(self):
+        st = Stats()
+        _check(self.lib.ppc_hip_wait(self.ctx, ctypes.byref(st)))
+        return st
+
+    def hits(self, with_photon=False, cap=None):
+        cap = self.max_hits if cap is None else cap
+        st = Stats()
+        _check(self.lib.ppc_hip_wait(self.ctx, ctypes.byref(st)))
+        m = int(min(st.stored, cap))
+        h = np.zeros(m, dtype=HIT_DTYPE)
+        ph = np.zeros(m, dtype=np.uint32)
+        n = ctypes.c_uint32()
+        _check(self.lib.ppc_hip_hits(self.ctx, h.ctypes.data_as(ctypes.c_void_p),
+                                     ph.ctypes.data_as(ctypes.c_void_p) if with_photon else None, m,
+                                     ctypes.byref(n)))
+        h = h[:n.value]
+        return (h, ph[:n.value]) if with_photon else h
+
+    def records(self):
+        cap = max(self.cfg.rec_max, 1)
+        r = np.zeros(cap, dtype=RECORD_DTYPE)
+        n = ctypes.c_uint32()
+        _check(self.lib.ppc_hip_records(self.ctx, r.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n)))
+        return r[:n.value]
+
+    def run(self, bunches, num=None):
+        self.launch(bunches, num)
+        return self.wait()

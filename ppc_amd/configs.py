@@ -1,0 +1,238 @@
+"""Synthetic ice/detector directories in ppc's own file formats.
+
+Each generator writes a complete ice directory (the files ``ini.cxx:578-1700``
+reads: ``cfg.txt``, ``icemodel.par``, ``icemodel.dat``, ``geo-f2k``, ``wv.dat``,
+``as.dat``, ``rnd.txt`` and, for the tilted ice, ``tilt.par``/``tilt.dat``) from
+a seed, so that the same workload can be given to the reference's CPU build and
+to this package.  Formats follow ``doc/index.rst:516-697``.
+
+Workloads
+---------
+``c1``  7 strings on a 125 m hexagon (centre string 1), 60 DOMs at 17 m spacing,
+        100 layers of 10 m (1450-2450 m), per-layer b_e(400) log-uniform in
+        [0.02, 0.1] 1/m and a(400) log-uniform in [0.005, 0.03] 1/m, HG g=0.9
+        with SAM fraction 0.45, no tilt, no anisotropy.  The 405 nm flasher is
+        selected with ``WFLA=405`` and ``FWID=10`` in the environment.
+``c2``  86 strings of the radius-5 hex lattice x 60 DOMs, 170 layers of 10 m
+        (1100-2800 m), 1-D tilt with 6 reference strings along 225 deg and
+        offsets within +-20 m, 8 % absorption anisotropy at a seeded azimuth,
+        wavelength spectrum ~ 1/lambda^2 x smooth acceptance over 300-600 nm.
+        ``eff`` is scaled so that a 100 TeV EM cascade gives a Poisson mean of
+        1e9 propagated photons at xR=5.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+# constants of the reference front-end (ini.cxx:31-36, ini.cxx:553, f2k.cxx:12,68-69)
+OMR = 0.16510
+DPPM = 2450.08
+OMAV = 0.335822
+ZOFF = 1948.07
+DIR1 = 9.3
+SPACING = 125.0
+DOM_DEPTH0 = 1448.5
+DOM_STEP = 17.0
+NDOM = 60
+EM_YIELD = 5.321 * 0.910 / 0.9216  # photons per GeV of EM cascade before efficiency
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(_HERE, "ice", "_gen")
+
+
+def mwc_multipliers(n, below=0xFFFFFFFF):
+    """The first n safe-prime MWC multipliers below ``below`` (descending)."""
+    from . import _native
+
+    lib = _native.xppc_lib()
+    out = (ctypes.c_uint32 * n)()
+    k = lib.ppc_mwc_multipliers(ctypes.c_uint32(below), ctypes.c_uint32(n), out)
+    if k != n:
+        raise RuntimeError("multiplier search returned %d of %d" % (k, n))
+    return np.frombuffer(out, dtype=np.uint32).copy()
+
+
+def _hex_sites(radius):
+    """Hex lattice sites (i, j) within hex distance ``radius``; lattice vectors at
+    9.3 and 69.3 deg so that the rows align with the 9.3/129.3 deg cell axes
+    (ini.cxx:6-7)."""
+    a1 = np.array([math.cos(math.radians(DIR1)), math.sin(math.radians(DIR1))])
+    a2 = np.array([math.cos(math.radians(DIR1 + 60)), math.sin(math.radians(DIR1 + 60))])
+    sites = []
+    for i in range(-radius, radius + 1):
+        for j in range(-radius, radius + 1):
+            k = -i - j
+            if max(abs(i), abs(j), abs(k)) <= radius:
+                p = SPACING * (i * a1 + j * a2)
+                sites.append((float(p[0]), float(p[1])))
+    return sites
+
+
+def _string_positions(nstr):
+    if nstr == 7:
+        sites = _hex_sites(1)
+    else:
+        sites = _hex_sites(5)
+    r = [math.hypot(x, y) for x, y in sites]
+    ang = [math.degrees(math.atan2(y, x)) % 360.0 for x, y in sites]
+    order = sorted(range(len(sites)), key=lambda k: (round(r[k], 3), round(ang[k], 3)))
+    return [sites[k] for k in order[:nstr]]
+
+
+def _write_geo(path, strings):
+    with open(path, "w") as f:
+        for s, (x, y) in enumerate(strings, start=1):
+            for d in range(1, NDOM + 1):
+                depth = DOM_DEPTH0 + DOM_STEP * (d - 1)
+                omid = (s << 16) | d
+                f.write("S%03dD%02d\t0x%012x\t%.2f\t%.2f\t%.2f\t%d\t%d\n" % (s, d, omid, x, y, -depth, s, d))
+
+
+def _write_ice(path, depths, be, ba):
+    with open(os.path.join(path, "icemodel.dat"), "w") as f:
+        for dp, b, a in zip(depths, be, ba):
+            f.write("%.2f %.8g %.8g 0\n" % (dp, b, a))
+    # alpha, kappa as in SPICE; A = 0 so that abs(400) = a(400) exactly (ini.cxx:1688)
+    with open(os.path.join(path, "icemodel.par"), "w") as f:
+        f.write("0.898608505726 0.027638472617\n")
+        f.write("1.084106802940 0.014470303431\n")
+        f.write("0.0 0.0\n")
+        f.write("6954.090332031250 973.426452636719\n")
+
+
+def _write_cfg(path, eff, azimuth=0.0, lk=(0.0, 0.0, 0.0), xr=5, sf=0.45, g=0.9):
+    lines = [
+        "# ppc configuration file: follow strict order below",
+        "%d     # over-R: DOM radius oversize scaling factor" % xr,
+        "%.9g  # overall DOM efficiency correction" % eff,
+        "%.6g  # 0=HG; 1=SAM" % sf,
+        "%.6g  # g=<cos(theta)>" % g,
+        "",
+        "%.6f # azimuth of major anisotropy axis (deg)" % azimuth,
+        "0.0   # magnitude of major anisotropy coefficient k1",
+        "0.0   # magnitude of minor anisotropy coefficient k2",
+        "",
+        "0.0   # hole ice radius in units of [DOM radius]",
+        "0.03  # hole ice effective scattering length [m]",
+        "100   # hole ice absorption length [m]",
+        "%.6g  # hole ice 0=HG; 1=SAM" % sf,
+        "%.6g  # hole ice g=<cos(theta)>" % g,
+        "",
+        "%.9f # new absorption anisotropy log k1" % lk[0],
+        "%.9f # new absorption anisotropy log k2" % lk[1],
+        "%.9f # new absorption anisotropy log kz" % lk[2],
+        "",
+        "0.0   # scaling for old absorption anisotropy",
+    ]
+    with open(os.path.join(path, "cfg.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _write_wv(path):
+    """CDF of 1/lambda^2 x a smooth acceptance bump over 300-600 nm."""
+    wl = np.arange(300.0, 601.0, 10.0)
+    fine = np.linspace(300.0, 600.0, 3001)
+    acc = np.exp(-0.5 * ((fine - 410.0) / 70.0) ** 2) + 0.05
+    dens = acc / fine ** 2
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (dens[1:] + dens[:-1]) * np.diff(fine))])
+    cdf /= cdf[-1]
+    vals = np.interp(wl, fine, cdf)
+    vals[0], vals[-1] = 0.0, 1.0
+    with open(os.path.join(path, "wv.dat"), "w") as f:
+        for c, w in zip(vals, wl):
+            f.write("%.8f %.1f\n" % (c, w))
+
+
+def _write_common(path, nrnd):
+    with open(os.path.join(path, "as.dat"), "w") as f:
+        f.write("1\n1\n")
+    rnd = mwc_multipliers(nrnd)
+    with open(os.path.join(path, "rnd.txt"), "w") as f:
+        f.write("\n".join(str(int(a)) for a in rnd) + "\n")
+    _write_wv(path)
+
+
+def write_c1(path, seed=1, nrnd=131072):
+    os.makedirs(path, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    depths = 1455.0 + 10.0 * np.arange(100)
+    be = np.exp(rng.uniform(math.log(0.02), math.log(0.1), 100))
+    ba = np.exp(rng.uniform(math.log(0.005), math.log(0.03), 100))
+    _write_geo(os.path.join(path, "geo-f2k"), _string_positions(7))
+    _write_ice(path, depths, be, ba)
+    _write_cfg(path, eff=1.0)
+    _write_common(path, nrnd)
+    return path
+
+
+def c2_tilt_strings(strings):
+    """Six reference strings along the 225 deg tilt gradient (tilt.par)."""
+    lnx, lny = math.cos(math.radians(225.0)), math.sin(math.radians(225.0))
+    proj = [lnx * x + lny * y for x, y in strings]
+    perp = [abs(-lny * x + lnx * y) for x, y in strings]
+    cand = sorted(range(len(strings)), key=lambda k: (perp[k] > 70.0, proj[k]))
+    near = [k for k in cand if perp[k] <= 70.0]
+    near.sort(key=lambda k: proj[k])
+    pick_idx = np.linspace(0, len(near) - 1, 6).round().astype(int)
+    pick = [near[i] for i in pick_idx]
+    zero = min(pick, key=lambda k: abs(proj[k]))
+    return [(k + 1, proj[k] - proj[zero]) for k in pick], zero + 1
+
+
+def write_c2(path, seed=2, nrnd=131072):
+    os.makedirs(path, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    strings = _string_positions(86)
+    depths = 1105.0 + 10.0 * np.arange(170)
+    be = np.exp(rng.uniform(math.log(0.02), math.log(0.1), 170))
+    ba = np.exp(rng.uniform(math.log(0.005), math.log(0.03), 170))
+    _write_geo(os.path.join(path, "geo-f2k"), strings)
+    _write_ice(path, depths, be, ba)
+    azimuth = float(rng.uniform(0.0, 360.0))
+    lk = math.log(1.08)
+    # Mean photons of a 1e5 GeV EM cascade: eff * ppm(~DPPM) * EM_YIELD * E / xR^2 (f2k.cxx:68-71,
+    # f2k.cxx:502, ini.cxx:1488); eff is set so that the mean is 1e9.
+    eff = 1e9 * 25.0 / (EM_YIELD * 1e5 * DPPM)
+    _write_cfg(path, eff=eff, azimuth=azimuth, lk=(2.0 * lk / 3.0, -lk / 3.0, -lk / 3.0))
+    tilt, _ = c2_tilt_strings(strings)
+    with open(os.path.join(path, "tilt.par"), "w") as f:
+        for s, d in tilt:
+            f.write("%d %.3f\n" % (s, d))
+    rows = 1100.0 + 10.0 * np.arange(171)
+    phase = rng.uniform(0, 2 * math.pi, 6)
+    amp = rng.uniform(8.0, 20.0, 6)
+    with open(os.path.join(path, "tilt.dat"), "w") as f:
+        for dp in rows:
+            offs = [amp[k] * math.sin(2 * math.pi * (dp - 1100.0) / 850.0 + phase[k]) * (0.6 + 0.4 * k / 5.0)
+                    for k in range(6)]
+            f.write("%.2f " % dp + " ".join("%.4f" % o for o in offs) + "\n")
+    _write_common(path, nrnd)
+    return path
+
+
+def c2_cascade(seed=2):
+    """The 100 TeV EM cascade of the 86-string workload: f2k TR line fields
+    (x, y, z in the detector frame, zenith/azimuth in degrees, E in GeV)."""
+    rng = np.random.default_rng(seed + 1000)
+    depth = 2000.0
+    x, y = rng.uniform(-60.0, 60.0, 2)
+    cz = rng.uniform(-1.0, 1.0)
+    th = math.degrees(math.acos(cz))
+    ph = rng.uniform(0.0, 360.0)
+    return dict(x=float(x), y=float(y), z=float(ZOFF - depth), th=float(th), ph=float(ph), E=1e5, t=0.0)
+
+
+_WRITERS = {"c1": write_c1, "c2": write_c2}
+
+
+def ensure(name, nrnd=131072):
+    """Path of the generated ice directory ``name`` (generated on first use)."""
+    path = os.path.join(CACHE, name)
+    stamp = os.path.join(path, ".complete")
+    if not os.path.exists(stamp):
+        _WRITERS[name](path, nrnd=nrnd)
+        with open(stamp, "w") as f:
+            f.write("ok\n")
+    return path

@@ -1,0 +1,82 @@
+// kparams.h -- the parameter block of the propagation kernel.
+//
+// Scalars travel as a by-value kernel argument (uniform values land in SGPRs
+// through the kernel-argument segment); the small per-layer, per-string and
+// cell tables are packed by the host into one contiguous image that every
+// workgroup copies into LDS at start; the large tables (DOM positions, 2-D
+// tilt maps, wavelength factors) stay in HBM and are reached through L2.
+#ifndef PPC_KPARAMS_H
+#define PPC_KPARAMS_H
+
+#include <stdint.h>
+
+#include "ppc_hip.h"
+
+namespace ppc {
+
+constexpr int kLMAX = PPC_HIP_LMAX;
+constexpr int kWNUM = PPC_HIP_WNUM;
+constexpr int kCX = PPC_HIP_CX, kCY = PPC_HIP_CY;
+constexpr int kCTY = PPC_HIP_CTY;
+
+// global counters of one launch
+struct Counters {
+  unsigned long long photons, steps, hits, killed, nonfinite;
+  unsigned int next;   // next photon to hand out (dynamic refill)
+  unsigned int hidx;   // next free hit slot
+  unsigned int nrec;   // records written
+  unsigned int pad[3];
+};
+
+struct KParams {
+  // ---- ice / detector scalars (names follow gpu/ini.cxx:452-518) ----
+  int size;
+  float dh, rdh, hmin;
+  float ocv, sf, g, gr, xR;
+  float SF, G, GR, hr, hr2, hs, ha, hifl;
+  float azx, azy, k1, k2, kz, fr;
+  float sum, bfr[12];
+  float rx;
+  int cn0, cn1;
+  float cl0, cl1, crst0, crst1;
+  float cb00, cb01, cb10, cb11;
+  int tmod, vthk, lnum, lpts;
+  float lmin, lrdz, r0, lnx, lny;
+  float mmin0, mmin1, mstp0, mstp1;
+  int mnum0, mnum1, mcut0, mcut1;
+  int nsc, gsize;
+  int uniform_dom;  // every DOM has R = OMR and F = 1
+
+  // ---- LDS image (byte offsets into it) ----
+  const uint4* image;
+  int image_words;  // 16-byte words
+  int off_lay, off_az, off_lr, off_mcol, off_is, off_ls, off_sc, off_lp;
+  int lp_in_lds, lp_stride;  // 1-D tilt table in LDS; row stride in float2
+
+  // ---- HBM tables ----
+  const float2* lp;         // [lnum][LYRS] (value, slope), used when not in LDS
+  const ppc_hip_wv* wv;     // [512]
+  const float4* dom;        // [gsize] x, y, z, R
+  const float* domF;        // [gsize]
+
+  // ---- RNG ----
+  const uint32_t* rm;  // multiplier per lane
+  uint2* rs;           // (x, c) per lane
+
+  // ---- launch ----
+  const ppc_hip_bunch* pz;
+  const uint32_t* pstart;  // exclusive prefix sums of bunch sizes, [nb+1]
+  uint32_t nb, num;
+  ppc_hip_hit* hits;
+  uint32_t* hit_photon;
+  uint32_t hnum;
+  Counters* ctr;
+  ppc_hip_record* rec;
+  uint32_t rec_stride, rec_max;
+  int refill, deterministic;
+  uint32_t lanes;
+};
+
+}  // namespace ppc
+
+#endif

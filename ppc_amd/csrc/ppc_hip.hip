@@ -1,0 +1,452 @@
+// ppc_hip.hip -- host side of the C-ABI in include/ppc_hip.h.
+//
+// Owns the device buffers of one GPU, packs the tables into the kernel's
+// parameter block and LDS image, and runs launches on one HIP stream with
+// events around the kernel (the roles of struct gpu in gpu/ppc.cu:102-321).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kparams.h"
+#include "ppc_hip.h"
+
+namespace ppc {
+hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream);
+}
+
+using ppc::KParams;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      return fail(e_ == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE, "%s: %s (%s:%d)", #call, \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, sizeof(T) * (n > 0 ? n : 1));
+}
+
+}  // namespace
+
+struct ppc_hip_ctx {
+  int device = 0;
+  int cus = 0;
+  ppc_hip_config cfg{};
+  int lanes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
+  bool launched = false, uploaded = false, copy_pending = false;
+
+  // launch buffers
+  ppc_hip_bunch* d_pz = nullptr;
+  uint32_t* d_pstart = nullptr;
+  ppc_hip_hit* d_hits = nullptr;
+  uint32_t* d_hit_photon = nullptr;
+  ppc::Counters* d_ctr = nullptr;
+  ppc_hip_record* d_rec = nullptr;
+  uint2* d_rs = nullptr;
+  uint32_t* d_rm = nullptr;
+  // pinned staging
+  ppc_hip_bunch* h_pz = nullptr;
+  uint32_t* h_pstart = nullptr;
+  ppc::Counters* h_ctr = nullptr;
+
+  // tables
+  uint4* d_image = nullptr;
+  float2* d_lp = nullptr;
+  ppc_hip_wv* d_wv = nullptr;
+  float4* d_dom = nullptr;
+  float* d_domF = nullptr;
+  size_t lds_bytes = 0;
+  KParams P{};
+  uint32_t last_num = 0;
+};
+
+extern "C" {
+
+const char* ppc_hip_last_error(void) { return g_err.c_str(); }
+
+int ppc_hip_device_count(int32_t* count) {
+  if (!count) return fail(PPC_HIP_E_ARG, "count is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  *count = n;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_device_info(int32_t device, char* name, size_t name_len, int32_t* cus, uint64_t* mem) {
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (name && name_len > 0) {
+    snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (mem) *mem = prop.totalGlobalMem;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
+  if (!out) return fail(PPC_HIP_E_ARG, "ctx is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(PPC_HIP_E_ARG, "device %d not present (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+
+  ppc_hip_ctx* c = new ppc_hip_ctx();
+  c->device = device;
+  c->cus = prop.multiProcessorCount;
+  if (cfg) c->cfg = *cfg;
+  ppc_hip_config& k = c->cfg;
+  if (k.threads <= 0) k.threads = 256;
+  if (k.threads % 64 != 0 || k.threads > 1024) {
+    delete c;
+    return fail(PPC_HIP_E_ARG, "threads per workgroup must be a multiple of 64 up to 1024 (got %d)", k.threads);
+  }
+  if (k.blocks <= 0) k.blocks = 2 * c->cus * (256 / k.threads > 0 ? 256 / k.threads : 1);
+  if (k.refill <= 0) k.refill = 16;
+  if (k.refill > 64) k.refill = 64;
+  if (k.max_bunches == 0) k.max_bunches = 1u << 20;
+  if (k.max_hits == 0) k.max_hits = 1u << 24;
+  c->lanes = k.blocks * k.threads;
+
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (r != hipSuccess && e == hipSuccess) e = r;
+  };
+  chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  chk(hipEventCreate(&c->ev0));
+  chk(hipEventCreate(&c->ev1));
+  chk(hipEventCreateWithFlags(&c->ev_copy, hipEventDisableTiming));
+  chk(dalloc(&c->d_pz, k.max_bunches));
+  chk(dalloc(&c->d_pstart, k.max_bunches + 1));
+  chk(dalloc(&c->d_hits, k.max_hits));
+  chk(dalloc(&c->d_hit_photon, k.max_hits));
+  chk(dalloc(&c->d_ctr, 1));
+  chk(dalloc(&c->d_rec, k.rec_max > 0 ? k.rec_max : 1));
+  chk(dalloc(&c->d_rs, c->lanes));
+  chk(dalloc(&c->d_rm, c->lanes));
+  chk(hipHostMalloc((void**)&c->h_pz, sizeof(ppc_hip_bunch) * k.max_bunches, hipHostMallocDefault));
+  chk(hipHostMalloc((void**)&c->h_pstart, sizeof(uint32_t) * (k.max_bunches + 1), hipHostMallocDefault));
+  chk(hipHostMalloc((void**)&c->h_ctr, sizeof(ppc::Counters), hipHostMallocDefault));
+  if (e != hipSuccess) {
+    int code = e == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE;
+    ppc_hip_close(c);
+    return fail(code, "allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_close(ppc_hip_ctx* c) {
+  if (!c) return PPC_HIP_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
+                  (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
+                  (void*)c->d_dom, (void*)c->d_domF})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr})
+    if (p) (void)hipHostFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_copy) (void)hipEventDestroy(c->ev_copy);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_lanes(const ppc_hip_ctx* c, int32_t* lanes) {
+  if (!c || !lanes) return fail(PPC_HIP_E_ARG, "NULL argument");
+  *lanes = c->lanes;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_capacity(const ppc_hip_ctx* c, uint32_t* max_bunches, uint32_t* max_hits) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  if (max_bunches) *max_bunches = c->cfg.max_bunches;
+  if (max_hits) *max_hits = c->cfg.max_hits;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
+  if (!c || !t) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (t->size < 1 || t->size > PPC_HIP_MAXLYS) return fail(PPC_HIP_E_ARG, "layers %d out of range", t->size);
+  if (t->gsize < 0 || t->nsc < 0 || t->nsc > PPC_HIP_NSTR) return fail(PPC_HIP_E_ARG, "bad geometry sizes");
+  if (t->rsize < c->lanes)
+    return fail(PPC_HIP_E_CAPACITY, "not enough multipliers: %d for %d lanes", t->rsize, c->lanes);
+  if (t->cn[0] < 0 || t->cn[0] > PPC_HIP_CX || t->cn[1] < 0 || t->cn[1] > PPC_HIP_CY)
+    return fail(PPC_HIP_E_ARG, "cell grid %dx%d exceeds %dx%d", t->cn[0], t->cn[1], PPC_HIP_CX, PPC_HIP_CY);
+  if (t->tmod != 0 && (t->lnum < 1 || t->lnum > PPC_HIP_LMAX || t->lpts < 0 || t->lpts + 1 > PPC_HIP_LYRS))
+    return fail(PPC_HIP_E_ARG, "tilt table %d x %d out of range", t->lnum, t->lpts);
+  if (t->tmod == 1 && t->lnum < 2) return fail(PPC_HIP_E_ARG, "1-d tilt needs at least 2 reference points");
+  if (t->tmod == 2) {
+    if (t->mnum[0] < 1 || t->mnum[0] > PPC_HIP_CTX || t->mnum[1] < 1 || t->mnum[1] > PPC_HIP_CTY)
+      return fail(PPC_HIP_E_ARG, "2-d tilt grid %dx%d out of range", t->mnum[0], t->mnum[1]);
+    for (int i = 0; i < t->mnum[0]; i++)
+      for (int j = 0; j < t->mnum[1]; j++)
+        if (t->mcol[i * PPC_HIP_CTY + j] >= t->lnum) return fail(PPC_HIP_E_ARG, "2-d tilt map index out of range");
+  }
+  if (t->tmod < 0 || t->tmod > 2) return fail(PPC_HIP_E_ARG, "tilt model %d unknown", t->tmod);
+  // every cell list entry must name a line, every line a DOM range inside the geometry
+  for (int i = 0; i < t->cn[0]; i++)
+    for (int j = 0; j < t->cn[1]; j++)
+      for (uint32_t k = t->is[i * PPC_HIP_CY + j], guard = 0; k != 0x8000u; guard++) {
+        if (k >= PPC_HIP_NSTR || guard > PPC_HIP_NSTR) return fail(PPC_HIP_E_ARG, "corrupt cell list");
+        uint32_t m = t->ls[k];
+        if ((int)(m & 0x7fff) >= t->nsc) return fail(PPC_HIP_E_ARG, "cell list names line %u of %d", m & 0x7fff, t->nsc);
+        k = m & 0x8000 ? 0x8000u : k + 1;
+      }
+  for (int i = 0; i < t->nsc; i++) {
+    const ppc_hip_line& s = t->sc[i];
+    if (s.n < 0 || s.max < 0 || s.n + s.max >= t->gsize || s.n > 0xffff || s.max > 0xffff)
+      return fail(PPC_HIP_E_ARG, "line %d: DOMs %d..%d outside 0..%d", i, s.n, s.n + s.max, t->gsize - 1);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+
+  KParams& P = c->P;
+  P = KParams{};
+  P.size = t->size, P.dh = t->dh, P.rdh = t->rdh, P.hmin = t->hmin;
+  P.ocv = t->ocv, P.sf = t->sf, P.g = t->g, P.gr = t->gr, P.xR = t->xR;
+  P.SF = t->SF, P.G = t->G, P.GR = t->GR, P.hr = t->hr, P.hr2 = t->hr2, P.hs = t->hs, P.ha = t->ha;
+  P.hifl = t->hifl;
+  P.azx = t->azx, P.azy = t->azy, P.k1 = t->k1, P.k2 = t->k2, P.kz = t->kz, P.fr = t->fr;
+  P.sum = t->sum;
+  for (int i = 0; i < 12; i++) P.bfr[i] = t->bfr[i];
+  P.rx = t->rx;
+  P.cn0 = t->cn[0], P.cn1 = t->cn[1];
+  P.cl0 = t->cl[0], P.cl1 = t->cl[1], P.crst0 = t->crst[0], P.crst1 = t->crst[1];
+  P.cb00 = t->cb[0], P.cb01 = t->cb[1], P.cb10 = t->cb[2], P.cb11 = t->cb[3];
+  P.tmod = t->tmod, P.vthk = t->vthk, P.lnum = t->lnum, P.lpts = t->lpts;
+  P.lmin = t->lmin, P.lrdz = t->lrdz, P.r0 = t->r0, P.lnx = t->lnx, P.lny = t->lny;
+  P.mmin0 = t->mmin[0], P.mmin1 = t->mmin[1], P.mstp0 = t->mstp[0], P.mstp1 = t->mstp[1];
+  P.mnum0 = t->mnum[0], P.mnum1 = t->mnum[1], P.mcut0 = t->mcut[0], P.mcut1 = t->mcut[1];
+  P.nsc = t->nsc, P.gsize = t->gsize;
+
+  // ---- LDS image ----
+  std::vector<uint8_t> img;
+  auto put = [&](const void* src, size_t bytes) {
+    size_t off = (img.size() + 15) & ~size_t(15);
+    img.resize(off + bytes);
+    if (bytes) memcpy(img.data() + off, src, bytes);
+    return (int)off;
+  };
+  P.off_lay = put(t->lay, sizeof(ppc_hip_layer) * t->size);
+  P.off_az = put(t->az, sizeof(ppc_hip_aniz) * t->size);
+  P.off_lr = put(t->lr, sizeof(float) * PPC_HIP_LMAX);
+  P.off_mcol = put(t->mcol, sizeof(int32_t) * PPC_HIP_CTX * PPC_HIP_CTY);
+  P.off_is = put(t->is, sizeof(uint16_t) * PPC_HIP_CX * PPC_HIP_CY);
+  P.off_ls = put(t->ls, sizeof(uint16_t) * PPC_HIP_NSTR);
+  {
+    std::vector<float4> sc(2 * std::max(t->nsc, 1));
+    for (int i = 0; i < t->nsc; i++) {
+      const ppc_hip_line& s = t->sc[i];
+      sc[2 * i] = make_float4(s.x, s.y, s.r, s.h);
+      uint32_t bits = (uint32_t)s.n | ((uint32_t)s.max << 16);
+      float fb;
+      memcpy(&fb, &bits, 4);
+      sc[2 * i + 1] = make_float4(s.d, s.dl, s.dh, fb);
+    }
+    P.off_sc = put(sc.data(), sizeof(float4) * sc.size());
+  }
+  // tilt table: rows k = 0..lpts are read (clamped), so lpts+1 entries per reference point
+  const int lp_stride = t->tmod != 0 ? t->lpts + 1 : 1;
+  const int lp_rows = t->tmod != 0 ? t->lnum : 0;
+  std::vector<float2> lp(std::max(lp_rows * lp_stride, 1));
+  for (int i = 0; i < lp_rows; i++)
+    for (int k = 0; k < lp_stride; k++)
+      lp[i * lp_stride + k] = make_float2(t->lp[(i * PPC_HIP_LYRS + k) * 2], t->lp[(i * PPC_HIP_LYRS + k) * 2 + 1]);
+  P.lp_stride = lp_stride;
+  P.lp_in_lds = lp.size() * sizeof(float2) <= 16384 ? 1 : 0;
+  if (P.lp_in_lds) P.off_lp = put(lp.data(), sizeof(float2) * lp.size());
+  img.resize((img.size() + 15) & ~size_t(15));
+  c->lds_bytes = img.size();
+  P.image_words = (int)(img.size() / 16);
+  if (c->lds_bytes > 64 * 1024) return fail(PPC_HIP_E_CAPACITY, "LDS image of %zu bytes too large", c->lds_bytes);
+
+  for (void* p : {(void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv, (void*)c->d_dom, (void*)c->d_domF})
+    if (p) (void)hipFree(p);
+  c->d_image = nullptr, c->d_lp = nullptr, c->d_wv = nullptr, c->d_dom = nullptr, c->d_domF = nullptr;
+  HIPCHK(dalloc(&c->d_image, P.image_words));
+  HIPCHK(hipMemcpy(c->d_image, img.data(), img.size(), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(&c->d_lp, lp.size()));
+  HIPCHK(hipMemcpy(c->d_lp, lp.data(), sizeof(float2) * lp.size(), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(&c->d_wv, PPC_HIP_WNUM));
+  HIPCHK(hipMemcpy(c->d_wv, t->wv, sizeof(ppc_hip_wv) * PPC_HIP_WNUM, hipMemcpyHostToDevice));
+  {
+    std::vector<float4> dom(std::max(t->gsize, 1));
+    std::vector<float> domF(std::max(t->gsize, 1), 1.f);
+    bool uniform = true;
+    for (int i = 0; i < t->gsize; i++) {
+      const ppc_hip_dom& d = t->oms[i];
+      dom[i] = make_float4(d.r[0], d.r[1], d.r[2], d.R);
+      domF[i] = d.F;
+      if (d.F != 1.f) uniform = false;
+    }
+    P.uniform_dom = uniform ? 1 : 0;
+    HIPCHK(dalloc(&c->d_dom, dom.size()));
+    HIPCHK(hipMemcpy(c->d_dom, dom.data(), sizeof(float4) * dom.size(), hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&c->d_domF, domF.size()));
+    HIPCHK(hipMemcpy(c->d_domF, domF.data(), sizeof(float) * domF.size(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c->d_rm, t->rm, sizeof(uint32_t) * c->lanes, hipMemcpyHostToDevice));
+
+  P.image = c->d_image;
+  P.lp = c->d_lp;
+  P.wv = c->d_wv;
+  P.dom = c->d_dom;
+  P.domF = c->d_domF;
+  P.rm = c->d_rm;
+  P.rs = c->d_rs;
+  P.pz = c->d_pz;
+  P.pstart = c->d_pstart;
+  P.hits = c->d_hits;
+  P.hit_photon = c->d_hit_photon;
+  P.hnum = c->cfg.max_hits;
+  P.ctr = c->d_ctr;
+  P.rec = c->d_rec;
+  P.rec_stride = c->cfg.rec_stride;
+  P.rec_max = c->cfg.rec_max;
+  P.refill = c->cfg.refill;
+  P.deterministic = c->cfg.deterministic;
+  P.lanes = (uint32_t)c->lanes;
+  c->uploaded = true;
+  return ppc_hip_seed(c, 0);
+}
+
+int ppc_hip_seed(ppc_hip_ctx* c, uint32_t sv) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  std::vector<uint64_t> st(c->lanes, (uint64_t)(uint32_t)(362436069u + sv) | ((uint64_t)1234567u << 32));
+  return ppc_hip_set_states(c, st.data(), c->lanes);
+}
+
+int ppc_hip_set_states(ppc_hip_ctx* c, const uint64_t* states, int32_t lanes) {
+  if (!c || !states || lanes != c->lanes) return fail(PPC_HIP_E_ARG, "states: need %d lanes", c ? c->lanes : 0);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_rs, states, sizeof(uint64_t) * lanes, hipMemcpyHostToDevice));
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_get_states(ppc_hip_ctx* c, uint64_t* states, int32_t lanes) {
+  if (!c || !states || lanes != c->lanes) return fail(PPC_HIP_E_ARG, "states: need %d lanes", c ? c->lanes : 0);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(states, c->d_rs, sizeof(uint64_t) * lanes, hipMemcpyDeviceToHost));
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  if (!c->uploaded) return fail(PPC_HIP_E_STATE, "launch before upload");
+  if (nb > c->cfg.max_bunches) return fail(PPC_HIP_E_CAPACITY, "%u bunches > capacity %u", nb, c->cfg.max_bunches);
+  if (num > 0 && (nb == 0 || !bunches)) return fail(PPC_HIP_E_ARG, "photons without bunches");
+  HIPCHK(hipSetDevice(c->device));
+  // the staging buffers may still feed the previous copy
+  if (c->copy_pending) HIPCHK(hipEventSynchronize(c->ev_copy));
+  uint64_t total = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    c->h_pstart[b] = (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
+    total += bunches[b].num;
+  }
+  if (total < num) return fail(PPC_HIP_E_ARG, "bunches hold %llu photons, launch asks for %u", (unsigned long long)total, num);
+  for (uint32_t b = 0; b < nb; b++) {  // DOM indices the kernel dereferences
+    const ppc_hip_bunch& p = bunches[b];
+    if (p.type > 0 && p.u.l.ka > 999.f && p.n[3] != 0 && (p.u.l.fla < 0 || p.u.l.fla >= c->P.gsize))
+      return fail(PPC_HIP_E_ARG, "bunch %u: flasher DOM %d outside the geometry", b, (int)p.u.l.fla);
+  }
+  c->h_pstart[nb] = (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
+  if (nb) memcpy(c->h_pz, bunches, sizeof(ppc_hip_bunch) * nb);
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipEventRecord(c->ev_copy, c->stream));
+  c->copy_pending = true;
+  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(ppc::Counters), c->stream));
+  KParams P = c->P;
+  P.nb = nb, P.num = num;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (num > 0) HIPCHK(ppc::launch_propagate(P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->launched = true;
+  c->last_num = num;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "wait without launch");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (st) {
+    const ppc::Counters& k = *c->h_ctr;
+    st->photons = k.photons;
+    st->steps = k.steps;
+    st->hits = k.hidx;
+    st->stored = std::min<uint64_t>(k.hidx, c->cfg.max_hits);
+    st->killed = k.killed;
+    st->nonfinite = k.nonfinite;
+    st->device_ms = ms;
+    st->lanes = c->lanes;
+  }
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_hits(ppc_hip_ctx* c, ppc_hip_hit* hits, uint32_t* photon, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "hits without launch");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  uint32_t stored = std::min<uint32_t>(c->h_ctr->hidx, c->cfg.max_hits);
+  uint32_t m = std::min(stored, cap);
+  if (m > 0 && hits) HIPCHK(hipMemcpy(hits, c->d_hits, sizeof(ppc_hip_hit) * m, hipMemcpyDeviceToHost));
+  if (m > 0 && photon) HIPCHK(hipMemcpy(photon, c->d_hit_photon, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  *n = m;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_records(ppc_hip_ctx* c, ppc_hip_record* rec, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "records without launch");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  uint32_t total = 0;
+  if (c->cfg.rec_stride > 0) total = std::min<uint32_t>((c->last_num + c->cfg.rec_stride - 1) / c->cfg.rec_stride, c->cfg.rec_max);
+  uint32_t m = std::min(total, cap);
+  if (m > 0 && rec) HIPCHK(hipMemcpy(rec, c->d_rec, sizeof(ppc_hip_record) * m, hipMemcpyDeviceToHost));
+  *n = m;
+  return PPC_HIP_OK;
+}
+
+}  // extern "C"

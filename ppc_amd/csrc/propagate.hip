@@ -1,0 +1,873 @@
+// propagate.hip -- photon propagation kernel for CDNA4 (gfx950).
+//
+// One persistent launch: every lane owns one multiply-with-carry stream and
+// propagates photons one after another.  Photons are handed out per wave: when
+// at least `refill` of the 64 lanes are idle, one lane claims that many photon
+// indices with a single atomic, and each idle lane emits its photon (source
+// sampling) before the wave steps again.  One loop iteration is one transport
+// step of the reference kernel (gpu/pro.cu:486-889): absorption anisotropy,
+// layer walk to the next scattering/absorption point, DOM search through the
+// cell grid, hit recording, birefringence, and scattering.  Hits are compacted
+// per wave with ballot + prefix count into the HBM hit buffer.
+//
+// Physics and the order in which each photon consumes its MWC stream follow
+// gpu/pro.cu:287-904 line by line, so that a photon replayed from its recorded
+// stream state on the CPU gives the same path.  The per-wavelength ice tables
+// are evaluated from per-layer factors held in LDS (sca = s0 + s1*la,
+// abs = p*lk + t*abl), instead of a 512 x layers table.
+#include <hip/hip_runtime.h>
+
+#include "kparams.h"
+
+namespace ppc {
+
+#define FPI 3.141592653589793f
+constexpr float XXX = 1.e-5f;
+constexpr float OMR = 0.16510f;
+
+// ---------------------------------------------------------------- math
+// Hardware transcendental instructions: v_log/v_exp are base 2, v_sin/v_cos
+// take revolutions; all are accurate to about 1 ulp on their domain here.
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.69314718056f; }
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
+__device__ __forceinline__ float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * frcp(b); }
+__device__ __forceinline__ float sq(float x) { return x * x; }
+__device__ __forceinline__ int rn(float x) { return (int)__builtin_rintf(x); }
+__device__ __forceinline__ int rd(float x) { return (int)__builtin_floorf(x); }
+__device__ __forceinline__ int ru(float x) { return (int)__builtin_ceilf(x); }
+
+// ---------------------------------------------------------------- RNG
+// Multiply-with-carry: t = a*x + c (one v_mad_u64_u32), x = lo(t), c = hi(t);
+// uniform in (0,1] from the top 23 bits of x (gpu/pro.cu:47-57).
+struct Mwc {
+  uint32_t x, c, a;
+  __device__ __forceinline__ float operator()() {
+    uint64_t t = (uint64_t)a * x + c;
+    x = (uint32_t)t;
+    c = (uint32_t)(t >> 32);
+    return 2.f - __uint_as_float((x >> 9) | 0x3f800000u);
+  }
+};
+
+// gamma(k) variate (gpu/pro.cu:59-85): Weibull rejection for k<1, Cheng otherwise
+__device__ float gamma_rnd(float k, Mwc& s) {
+  float x;
+  if (k < 1) {
+    float c = frcp(k);
+    float d = (1 - k) * fpow(k, frcp(c - 1));
+    float z, e;
+    do {
+      z = -flog(s());
+      e = -flog(s());
+      x = fpow(z, c);
+    } while (z + e < d + x);
+  } else {
+    float b = k - 1.38629436112f;  // log 4
+    float l = fsqrt(2 * k - 1);
+    float c = 1 + 1.50407739678f;  // 1 + log 4.5
+    float u, v, y, z, r;
+    do {
+      u = s();
+      v = s();
+      y = -flog(frcp(v) - 1) * frcp(l);
+      x = k * fexp(y);
+      z = u * v * v;
+      r = b + (k + l) * y - x;
+    } while (r < 4.5f * z - c && r < flog(z));
+  }
+  return x;
+}
+
+// Box-Muller: radius from the first draw, angle from the second (gpu/pro.cu:87-95)
+__device__ __forceinline__ float gauss_rnd(Mwc& s) {
+  float r = fsqrt(-2 * flog(s()));
+  return r * __builtin_amdgcn_sinf(s());
+}
+__device__ __forceinline__ float2 gauss2_rnd(Mwc& s) {
+  float r = fsqrt(-2 * flog(s()));
+  float u = s();
+  return make_float2(r * __builtin_amdgcn_cosf(u), r * __builtin_amdgcn_sinf(u));
+}
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+  float r = frsq(x * x + y * y + z * z);
+  x *= r, y *= r, z *= r;
+}
+
+// Rotate unit n by the polar angle with 1-cos = cs, sin = si, around a uniform
+// azimuth (gpu/pro.cu:111-153); the largest component is swapped to x first.
+__device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
+  float nx = n.x, ny = n.y, nz = n.z;
+  float r0 = nx * nx, r1 = ny * ny, r2 = nz * nz;
+  int ax = 0;
+  if (r1 > r2) {
+    if (r1 > r0) {
+      float t = nx; nx = ny; ny = t;
+      t = r0; r0 = r1; r1 = t;
+      ax = 1;
+    }
+  } else if (r2 > r0) {
+    float t = nx; nx = nz; nz = t;
+    t = r0; r0 = r2; r2 = t;
+    ax = 2;
+  }
+  float a = frsq(r0 + r1);
+  float p1x = -ny * a, p1y = nx * a, p1z = 0;
+  float b = frsq(r0 + r2);
+  float p2x = -nz * b, p2y = 0, p2z = nx * b;
+  float qx = p1x - p2x, qy = p1y - p2y, qz = p1z - p2z;
+  p2x += p1x, p2y += p1y, p2z += p1z;
+  float w = frsq(qx * qx + qy * qy + qz * qz);
+  p1x = qx * w, p1y = qy * w, p1z = qz * w;
+  w = frsq(p2x * p2x + p2y * p2y + p2z * p2z);
+  p2x *= w, p2y *= w, p2z *= w;
+  float u = s();
+  float ps = __builtin_amdgcn_sinf(u), pc = __builtin_amdgcn_cosf(u);
+  nx -= cs * nx - si * (pc * p1x + ps * p2x);
+  ny -= cs * ny - si * (pc * p1y + ps * p2y);
+  nz -= cs * nz - si * (pc * p1z + ps * p2z);
+  normalize3(nx, ny, nz);
+  if (ax == 1) {
+    float t = nx; nx = ny; ny = t;
+  } else if (ax == 2) {
+    float t = nx; nx = nz; nz = t;
+  }
+  n.x = nx, n.y = ny, n.z = nz;
+}
+
+// ---------------------------------------------------------------- tables
+
+struct Tabs {
+  const float4* lay;   // s0, s1, p, t
+  const float4* az;    // k1, k2, ra, rb
+  const float* lr;
+  const int* mcol;
+  const uint16_t* is;
+  const uint16_t* ls;
+  const float4* sc;    // 2 per line: (x, y, r, h), (d, dl, dh, n | max<<16)
+  const float2* lp;
+  int lp_stride;
+};
+
+// wavelength factors of the photon being propagated
+struct Wave {
+  float la, lk, abl;
+  __device__ __forceinline__ float sca(const Tabs& T, int J) const {
+    float4 l = T.lay[J];
+    return l.x + l.y * la;
+  }
+  __device__ __forceinline__ float abs(const Tabs& T, int J) const {
+    float4 l = T.lay[J];
+    return l.z * lk + l.w * abl;
+  }
+};
+
+// depth shift of the ice layers (gpu/pro.cu:164-264)
+template <int TILT>
+__device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const float4& r, float& dz) {
+  if constexpr (TILT == 1) {
+    float nr = P.lnx * r.x + P.lny * r.y - P.r0;
+    int j = 1;
+    for (; j < kLMAX; j++)
+      if (nr < T.lr[j] || j == P.lnum - 1) break;
+    float z = (r.z - P.lmin) * P.lrdz;
+    int i = j - 1, k = min(max(rd(z), 0), P.lpts);
+    float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
+    float dw = T.lr[j] - T.lr[i];
+    float wx = fdiv(nr - T.lr[i], dw), wy = fdiv(T.lr[j] - nr, dw);
+    float qx = pj.x * wx + pi.x * wy;
+    float qy = pj.y * wx + pi.y * wy;
+    if (P.vthk != 0) dz = fdiv(dz, 1 - P.lrdz * qy);
+    return qx + qy * (z - k);
+  } else if constexpr (TILT == 2) {
+    float qx = P.cb00 * r.x + P.cb10 * r.y;
+    float qy = P.cb01 * r.x + P.cb11 * r.y;
+    qx = (qx - P.mmin0) / P.mstp0;
+    qy = (qy - P.mmin1) / P.mstp1;
+    {  // extrapolate outside the hexagon of the map
+      float x = qx, y = qy, xy;
+      int n0 = P.mnum0 - 1, n1 = P.mnum1 - 1, c0 = P.mcut0 - 1, c1 = P.mcut1 - 1;
+      if (y < 0 && x - y > 0 && x <= c0) x = (x - y) * c0 / (c0 - y), y = 0;
+      if (x < 0 && y - x >= 0 && y < c1) y = (y - x) * c1 / (c1 - x), x = 0;
+      if (y > n1 && x >= n1 - c1 && y - x > n1 - n0)
+        x = n1 - c1 + (n0 - (n1 - c1)) * (x - (n1 - c1)) / (y - (n1 - n0) - (n1 - c1)), y = n1;
+      if (x > n0 && y > n0 - c0 && y - x <= n1 - n0)
+        y = n0 - c0 + (n1 - (n0 - c0)) * (y - (n0 - c0)) / (x - (n0 - n1) - (n0 - c0)), x = n0;
+      if (y - x > c1 && y >= c1 && x < n1 - c1) {
+        xy = y - x - c1;
+        xy = c1 + (x + y - c1 + xy) * (n1 - c1) / (n1 - c1 + xy);
+        x = (xy - c1) / 2, y = (xy + c1) / 2;
+      }
+      if (x - y > c0 && x > c0 && y <= n0 - c0) {
+        xy = x - y - c0;
+        xy = c0 + (x + y - c0 + xy) * (n0 - c0) / (n0 - c0 + xy);
+        x = (xy + c0) / 2, y = (xy - c0) / 2;
+      }
+      qx = x, qy = y;
+    }
+    int nx = min(max(rd(qx), 0), P.mnum0);
+    int ny = min(max(rd(qy), 0), P.mnum1);
+    float fx = qx - nx, fy = qy - ny;
+    float qrx, qry;
+    int qnx, qny;
+    if (fx > fy)
+      qrx = fx, qry = fy, qnx = nx + 1, qny = ny;
+    else
+      qrx = fy, qry = fx, qnx = nx, qny = ny + 1;
+    auto col = [&](int a, int b) {
+      return a >= 0 && a < P.mnum0 && b >= 0 && b < P.mnum1 ? T.mcol[a * kCTY + b] : -1;
+    };
+    int cx = col(nx, ny), cy = col(qnx, qny), cz = col(nx + 1, ny + 1);
+    float wx = 1 - qrx, wy = qrx - qry, wz = qry;
+    float z = (r.z - P.lmin) * P.lrdz;
+    int k = min(max(rd(z), 0), P.lpts);
+    float ax = 0, ay = 0;
+    if (cx >= 0) {
+      float2 v = T.lp[cx * T.lp_stride + k];
+      ax += v.x * wx, ay += v.y * wx;
+    }
+    if (cy >= 0) {
+      float2 v = T.lp[cy * T.lp_stride + k];
+      ax += v.x * wy, ay += v.y * wy;
+    }
+    if (cz >= 0) {
+      float2 v = T.lp[cz * T.lp_stride + k];
+      ax += v.x * wz, ay += v.y * wz;
+    }
+    if (P.vthk != 0) dz = fdiv(dz, 1 - P.lrdz * ay);
+    return ax + ay * (z - k);
+  } else {
+    return 0.f;
+  }
+}
+
+// ---------------------------------------------------------------- lane state
+
+struct Photon {
+  float4 r, n;  // n.w = 1/group velocity
+  float TOT, SCA;
+  int old, ofla;
+  uint32_t seg, wbin, idx;
+  Wave w;
+};
+
+// Emit photon i: pick its bunch, a wavelength bin, and the source-specific
+// start and direction (gpu/pro.cu:334-477).  Returns the starting DOM (fla).
+__device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
+  // bunch: last b with pstart[b] <= i
+  uint32_t lo = 0, hi = P.nb;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (P.pstart[mid] <= i) lo = mid; else hi = mid;
+  }
+  const ppc_hip_bunch& p = P.pz[lo];
+
+  int j = min(rd(kWNUM * s()), kWNUM - 1);
+  const ppc_hip_wv wv = P.wv[j];
+  float4 r = make_float4(p.r[0], p.r[1], p.r[2], p.r[3]);
+  float4 n = make_float4(p.n[0], p.n[1], p.n[2], wv.ocm);
+  float l = p.n[3];
+  int fla, ofla;
+
+  if (p.type > 0) {  // flashers and standard candles
+    fla = p.u.l.fla, ofla = p.u.l.ofla;
+    const float ka = p.u.l.ka, up = p.u.l.up, fldr = p.u.l.fldr;
+    if (l == 0) {  // LED ring: uniform azimuth or N-fold pattern
+      float xi = s();
+      if (fldr < 0)
+        xi *= 2 * FPI;
+      else {
+        int rr = rd(fldr / 360) + 1;
+        int ss = rd(xi * rr);
+        xi = (fldr + ss * 360 / rr) * (FPI / 180.f);
+      }
+      sincosf(xi, &r.y, &r.x);
+      float cu;
+      sincosf(up, &n.z, &cu);
+      n.x = cu * r.x;
+      n.y = cu * r.y;
+    }
+    if (ka != 0) {
+      float xi;
+      if (ka > 999.f) {  // lab-measured beam profile
+        xi = s();
+        float s1 = 0.0191329f, s2 = 0.0686944f, C1 = 0.583583f, C2 = 0.967762f;
+        if (p.type == 1) s1 = 0.0196242f, s2 = 0.0750278f, C1 = 0.606833f, C2 = 0.960863f;
+        if (p.type == 2) s1 = 0.0185603f, s2 = 0.0624481f, C1 = 0.553192f, C2 = 0.974129f;
+        xi = xi < C1 ? s1 * fabsf(gauss_rnd(s)) : xi < C2 ? -s2 * flog(s()) : 2.f;
+        if (xi >= 2.f) xi = 2 * (1 - fsqrt(s()));
+      } else if (ka > 0)  // 2-d gaussian
+        xi = fminf(-ka * flog(1 - (1 - s()) * (1 - expf(-2 / ka))), 2.f);
+      else if (ka == -1)  // isotropic
+        xi = 2 * (1 - s());
+      else  // linear cone
+        xi = 2 * (1 - fsqrt(s()));
+      rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+    }
+    if (ka > 999.f) {  // start on the flasher board inside the DOM
+      const float FLZ = 0.07735f, FLR = 0.119f - 0.008f, FLB = 0.03396f;
+      if (l == 0) r.x *= FLR, r.y *= FLR, r.z = FLZ, r.w = 0;
+      float b = r.x * n.x + r.y * n.y + r.z * n.z;
+      float c = FLZ * FLZ + FLR * FLR - OMR * OMR;
+      float t = sqrtf(b * b - c) - b;
+      r.x += t * n.x, r.y += t * n.y, r.z += t * n.z, r.w += t * P.ocv;
+      if (fabsf(r.z) < FLB)
+        ofla = -2;
+      else if (l == 0 && r.z < 0 && s() < 0.9f)
+        ofla = -3;
+      if (ka > 1050.f) {  // cable shadow
+        const float rc = 0.023f, dc = OMR + rc;
+        float phc = (ka - 1080.f) * (FPI / 180.f);
+        float drx = r.x - dc * cosf(phc), dry = r.y - dc * sinf(phc);
+        float a = n.x * n.x + n.y * n.y;
+        if (a > 0) {
+          float b2 = n.x * drx + n.y * dry;
+          float c2 = drx * drx + dry * dry - rc * rc;
+          float D = b2 * b2 - a * c2;
+          if (D >= 0 && (-b2 + sqrtf(D)) / a > 0) ofla = -4;
+        }
+      }
+      if (l == 0)
+        r.x += p.r[0], r.y += p.r[1], r.z += p.r[2], r.w += p.r[3];
+      else {
+        float4 d = P.dom[fla];
+        r.x += d.x, r.y += d.y, r.z += d.z, r.w += p.r[3];
+      }
+    } else if (fla >= 0) {  // start on the DOM surface
+      r.x = OMR * n.x + p.r[0], r.y = OMR * n.y + p.r[1], r.z = OMR * n.z + p.r[2];
+      r.w = OMR * P.ocv + p.r[3];
+    } else
+      r = make_float4(p.r[0], p.r[1], p.r[2], p.r[3]);
+  } else {  // particles: muon segments and cascades
+    fla = -1, ofla = -1;
+    const float pa = p.u.p.a, pb = p.u.p.b, beta0 = p.u.p.beta, tau = p.u.p.tau;
+    if (l > 0)
+      l *= s();
+    else
+      l = pb * gamma_rnd(pa, s);
+    if (l > 0) {
+      r.x += l * n.x, r.y += l * n.y, r.z += l * n.z;
+      r.w += l * P.ocv / beta0;
+    }
+    if (tau > 0) {  // isotropic delayed emission
+      r.w -= tau * flog(s());
+      float xi = 1 - s();
+      rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+    } else {
+      float cs = wv.coschr, si = wv.sinchr;
+      if (p.f < s()) {  // cascade particle directions
+        const float a = 0.39f, b = 2.61f;
+        const float I = 1 - expf(-b * exp2f(a));
+        float xi = fminf(fpow(-flog(1 - s() * I) / b, 1 / a), 2.f);
+        rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+      } else {
+        float beta = beta0;
+        if (p.type < 0) {  // muon end point
+          const float ar = 0.26f * 0.9216f / 0.105658389f;
+          float dx = ar * (p.n[3] - l);
+          beta = sqrtf(dx * (2 + dx)) / (1 + dx);
+          if (beta >= cs) r.w -= P.ocv * (sqrtf(dx * dx + 1) - asinhf(1 / dx) - dx) / ar;
+        }
+        if (beta < 1) {
+          float xi = cs / beta;
+          if (xi <= 1) {
+            float sx = sqrtf(1 - xi * xi);
+            if (p.type < 0 && sx < s() * si) ofla = -3;
+            cs = xi, si = sx;
+          } else
+            ofla = -2;
+        }
+      }
+      rotate(1 - cs, si, n, s);  // Cherenkov cone
+    }
+  }
+  ph.r = r, ph.n = n;
+  ph.seg = p.q;
+  ph.wbin = wv.wvl;
+  ph.w.la = wv.la, ph.w.lk = wv.lk, ph.w.abl = wv.abl;
+  ph.ofla = ofla;
+  return fla;
+}
+
+// ---------------------------------------------------------------- kernel
+
+template <int TILT, bool EXT>
+__global__ __launch_bounds__(1024) void propagate(const KParams P) {
+  extern __shared__ uint4 lds[];
+  for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
+  __syncthreads();
+
+  const char* base = reinterpret_cast<const char*>(lds);
+  Tabs T;
+  T.lay = reinterpret_cast<const float4*>(base + P.off_lay);
+  T.az = reinterpret_cast<const float4*>(base + P.off_az);
+  T.lr = reinterpret_cast<const float*>(base + P.off_lr);
+  T.mcol = reinterpret_cast<const int*>(base + P.off_mcol);
+  T.is = reinterpret_cast<const uint16_t*>(base + P.off_is);
+  T.ls = reinterpret_cast<const uint16_t*>(base + P.off_ls);
+  T.sc = reinterpret_cast<const float4*>(base + P.off_sc);
+  T.lp = P.lp_in_lds ? reinterpret_cast<const float2*>(base + P.off_lp) : P.lp;
+  T.lp_stride = P.lp_stride;
+
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  Mwc s;
+  {
+    uint2 st = P.rs[gid];
+    s.x = st.x, s.c = st.y, s.a = P.rm[gid];
+  }
+
+  Photon ph;
+  ph.TOT = 0, ph.SCA = 0, ph.old = -1, ph.ofla = -1;
+  bool live = false;
+  bool done = false;  // no more photons to claim (wave-uniform)
+  uint32_t next_static = gid;
+  uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
+
+  while (true) {
+    // -------- refill idle lanes --------
+    if (!done) {
+      const uint64_t idle = __ballot(!live);
+      const int nidle = __popcll(idle);
+      if (nidle >= P.refill || nidle == __popcll(__ballot(true))) {
+        uint32_t i = 0xffffffffu;
+        if (P.deterministic) {
+          if (!live) i = next_static, next_static += P.lanes;
+          done = !__any(next_static < P.num);
+        } else {
+          uint32_t first = 0;
+          const int leader = __ffsll((unsigned long long)idle) - 1;
+          if ((int)lane == leader) first = atomicAdd(&P.ctr->next, (uint32_t)nidle);
+          first = __shfl(first, leader);
+          const uint32_t rank = __popcll(idle & ((1ull << lane) - 1));
+          if (!live) i = first + rank;
+          if ((unsigned long long)first + nidle >= P.num) done = true;
+        }
+        if (!live && i < P.num) {
+          if (P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
+            ppc_hip_record rc;
+            rc.photon = i, rc.x = s.x, rc.c = s.c, rc.a = s.a;
+            P.rec[i / P.rec_stride] = rc;
+          }
+          ph.idx = i;
+          int fla = emit(P, i, s, ph);
+          n_photons++;
+          ph.TOT = -flog(s());
+          ph.SCA = 0;
+          ph.old = fla;
+          if (ph.ofla < -1) {  // killed at emission: the reference still draws SCA
+            s();
+            ph.TOT = 0;
+            n_killed++;
+          } else
+            live = true;
+        }
+      }
+    }
+    if (!__any(live)) {
+      if (done) break;
+      continue;
+    }
+    if (live) {
+    // -------- one transport step (gpu/pro.cu:486-889) --------
+    n_steps++;
+    float4 r = ph.r, n = ph.n;
+    float TOT = ph.TOT, SCA = ph.SCA;
+    int old = ph.old;
+    const int ofla = ph.ofla;
+    const Wave w = ph.w;
+
+    if (SCA == 0) SCA = -flog(s());  // old keeps the DOM of the previous step / the emitter
+    float sca, tot;
+    const int sign = n.z < 0 ? -1 : 1;
+    const float anz = sign * n.z;
+    float edh = P.dh;
+    float z = r.z - zshift<TILT>(P, T, r, edh);
+    float nr = 1.f;
+    int I, J;
+    {
+      z = (z - P.hmin) * P.rdh;
+      I = min(max(rn(z), 0), P.size - 1);
+      if (TOT > 0) {  // absorption anisotropy
+        float n1 = P.azx * n.x + P.azy * n.y;
+        float n2 = -P.azy * n.x + P.azx * n.y;
+        float n3 = n.z;
+        if (EXT && P.fr != 0) {
+          float4 a = T.az[I];
+          float k1 = a.x, k2 = a.y;
+          float kz = frcp(k1 * k2);
+          float s1 = n1 * n1, l1 = k1 * k1;
+          float s2 = n2 * n2, l2 = k2 * k2;
+          float s3 = n3 * n3, l3 = kz * kz;
+          float B2 = nr / l1 + nr / l2 + nr / l3;
+          float nB = s1 / l1 + s2 / l2 + s3 / l3;
+          float An = s1 * l1 + s2 * l2 + s3 * l3;
+          nr = fpow((B2 - nB) * An / 2, P.fr);
+        }
+        n1 *= P.k1, n2 *= P.k2, n3 *= P.kz;
+        nr *= frsq(n1 * n1 + n2 * n2 + n3 * n3);
+        TOT = fdiv(TOT, nr);
+      }
+      float ahx = edh * (0.5f - sign * (z - I));
+      J = I;
+      sca = 0, tot = 0;
+      float ais = SCA, aia = TOT;
+      const float ranz = frcp(anz);
+      while (true) {  // layer walk
+        const float bs = w.sca(T, J), ba = w.abs(T, J);
+        float dsca = fdiv(ais, bs), dtot = fdiv(aia, ba);
+        float dist = fminf(dsca, dtot) * anz;
+        if ((sign < 0 ? J > 0 : J < P.size - 1) && dist > ahx) {
+          float del = ahx * ranz;
+          sca += del, tot += del;
+          ais -= bs * del, aia -= ba * del;
+          ahx = edh;
+          J += sign;
+        } else {
+          sca += dsca, tot += dtot;
+          break;
+        }
+      }
+      if (sca < 0) sca = 0;
+      if (tot < 0) tot = 0;
+      if (tot < sca)
+        sca = tot, tot = 0;
+      else
+        tot = (tot - sca) * w.abs(T, J);
+    }
+
+    int om = -1;
+    float del = sca, sqd = 0;
+    float hi = sca, hf = 0;
+    {  // DOM and hole-ice search in the cells overlapping the step
+      float rfx = r.x + sca * n.x, rfy = r.y + sca * n.y;
+      float pix = P.cb00 * r.x + P.cb10 * r.y, piy = P.cb01 * r.x + P.cb11 * r.y;
+      float pfx = P.cb00 * rfx + P.cb10 * rfy, pfy = P.cb01 * rfx + P.cb11 * rfy;
+      float lox = fminf(pix, pfx) - P.rx, hix = fmaxf(pix, pfx) + P.rx;
+      float loy = fminf(piy, pfy) - P.rx, hiy = fmaxf(piy, pfy) + P.rx;
+      int xlx = min(max(rn((lox - P.cl0) * P.crst0), 0), P.cn0);
+      int xhx = max(min(rn((hix - P.cl0) * P.crst0), P.cn0 - 1), -1);
+      int xly = min(max(rn((loy - P.cl1) * P.crst1), 0), P.cn1);
+      int xhy = max(min(rn((hiy - P.cl1) * P.crst1), P.cn1 - 1), -1);
+      const float np = 1 - n.z * n.z;
+      for (int ci = xlx; ci <= xhx; ci++)
+        for (int cj = xly; cj <= xhy; cj++)
+          for (uint32_t k = T.is[ci * kCY + cj]; k != 0x8000u;) {
+            const uint32_t m = T.ls[k];
+            k = (m & 0x8000u) ? 0x8000u : k + 1;
+            const float4 la = T.sc[2 * (m & 0x7fffu)];
+            float b = 0, c = 0, dr;
+            dr = la.x - r.x;
+            b += n.x * dr, c += dr * dr;
+            dr = la.y - r.y;
+            b += n.y * dr, c += dr * dr;
+            float D = b * b - (c - la.z * la.z) * np;
+            if (D >= 0) {
+              D = fsqrt(D);
+              float h1 = b - D, h2 = b + D;
+              if (h2 >= 0 && h1 <= sca * np) {
+                const float4 lb = T.sc[2 * (m & 0x7fffu) + 1];
+                if (np > XXX) {
+                  const float rnp = frcp(np);
+                  h1 *= rnp, h2 *= rnp;
+                  if (h1 < 0) h1 = 0;
+                  if (h2 > sca) h2 = sca;
+                } else
+                  h1 = 0, h2 = sca;
+                h1 = r.z + n.z * h1, h2 = r.z + n.z * h2;
+                float zl = n.z > 0 ? h1 : h2, zh = n.z > 0 ? h2 : h1;
+                const uint32_t bits = __float_as_uint(lb.w);
+                const int sn = bits & 0xffff, smax = bits >> 16;
+                int n1 = sn + min(smax + 1, max(0, ru(-(zh - lb.y - la.w) * lb.x)));
+                int n2 = sn + max(-1, min(smax, rd(-(zl - lb.z - la.w) * lb.x)));
+                for (int l = n1; l <= n2; l++) {
+                  if (l == old) continue;
+                  const float4 dm = P.dom[l];
+                  const float F = P.uniform_dom ? 1.f : P.domF[l];
+                  float a = 0, bb = 0, cc = 0, d2;
+                  const float f = F > 0 ? sq(frcp(F)) : 0;
+                  d2 = dm.x - r.x;
+                  a += n.x * n.x, bb += n.x * d2, cc += d2 * d2;
+                  d2 = dm.y - r.y;
+                  a += n.y * n.y, bb += n.y * d2, cc += d2 * d2;
+                  d2 = dm.z - r.z;
+                  a += f * n.z * n.z, bb += f * n.z * d2, cc += f * d2 * d2;
+                  if (a > 0) {
+                    const float ra = frcp(a);
+                    bb *= ra;
+                    const float xR = (l == ofla ? 1.f : P.xR);
+                    float DD = bb * bb - (cc - sq(xR * dm.w)) * ra;
+                    if (DD >= 0) {
+                      sqd = fsqrt(DD);
+                      float h = bb - sqd;
+                      if (F > 0 || fabsf(d2 - n.z * h) < -xR * F * dm.w) {
+                        sqd *= frcp(xR);
+                        h = bb - sqd;
+                        if (h > 0 && h <= del) om = l, del = h;
+                      }
+                    }
+                  }
+                }
+              }
+            }
+            if (EXT && P.hr > 0 &&
+                (P.hifl == 0 || (P.hifl < 0 && r.w < -P.hifl) || (P.hifl > 0 && r.w > P.hifl))) {
+              float D2 = b * b - (c - P.hr2) * np;
+              if (D2 > 0) {
+                D2 = fsqrt(D2);
+                float h1 = b - D2, h2 = b + D2;
+                if (h2 >= 0 && h1 <= sca * np) {
+                  if (np > XXX) {
+                    h1 /= np, h2 /= np;
+                    if (h1 < 0) h1 = 0;
+                    if (h2 > sca) h2 = sca;
+                  } else
+                    h1 = 0, h2 = sca;
+                  if (h1 < hi && h2 > 0.0031622776f * P.hr) hi = h1, hf = h2;
+                }
+              }
+            }
+          }
+    }
+
+    const float fin = fminf(del, hi);
+    const bool hole = fin < sca;
+    if (hole) {  // step cut short by a DOM or the hole-ice column: spend the optical depths
+      float xs = 0, xa = 0;
+      const float y = z + n.z * fin / edh;
+      J = min(max(rn(y), 0), P.size - 1);
+      if (I == J)
+        xs = fin * w.sca(T, I), xa = fin * w.abs(T, I);
+      else {
+        const float h = 0.5f - sign * (z - I);
+        const float g = 0.5f + sign * (y - J);
+        xs = h * w.sca(T, I) + g * w.sca(T, J);
+        xa = h * w.abs(T, I) + g * w.abs(T, J);
+        for (int i = I + sign; i != J; i += sign) xs += w.sca(T, i), xa += w.abs(T, i);
+        const float f = edh / anz;
+        xs *= f, xa *= f;
+      }
+      if (xs < 0) xs = 0;
+      if (xa < 0) xa = 0;
+      SCA -= xs, TOT -= xa;
+      TOT *= nr;
+      if (EXT && hi < del) {  // inside the hole ice
+        float fin2 = fminf(hi + fminf(SCA / P.hs, TOT / P.ha), hf);
+        if (om != -1) {
+          if (fin2 < del) del = fin2, om = -1;
+        } else {
+          if (fin2 < del)
+            del = fin2;
+          else
+            del = fminf(fin2, sca);
+        }
+        fin2 = del - hi;
+        SCA -= fin2 * P.hs, TOT -= fin2 * P.ha;
+      }
+    } else
+      SCA = 0, TOT = tot * nr;
+
+    r.x += del * n.x, r.y += del * n.y, r.z += del * n.z, r.w += del * n.w;
+
+    if (!isfinite(TOT) || !isfinite(SCA)) TOT = 0, om = -1, n_nonfinite++;
+
+    if (om != -1) {  // DOM hit
+      if (om == ofla)
+        TOT = 0;
+      else {
+        ppc_hip_hit h;
+        h.i = om, h.t = r.w, h.n = ph.seg, h.z = ph.wbin;
+        h.pth = acosf(n.z);
+        h.pph = atan2f(n.y, n.x);
+        sqd *= P.xR - 1;
+        const float4 dm = P.dom[om];
+        const float F = P.uniform_dom ? 1.f : P.domF[om];
+        float dx = dm.x - r.x + sqd * n.x;
+        float dy = dm.y - r.y + sqd * n.y;
+        float dz = dm.z - r.z + sqd * n.z;
+        h.dth = acosf(fminf(fmaxf(dz / (P.xR * dm.w * F), -1.f), 1.f));
+        h.dph = atan2f(dy, dx);
+        // wave-level compaction of the lanes that hit in this step
+        const uint64_t hm = __ballot(true);
+        const int leader = __ffsll((unsigned long long)hm) - 1;
+        uint32_t first = 0;
+        if ((int)lane == leader) {
+          first = atomicAdd(&P.ctr->hidx, (uint32_t)__popcll(hm));
+        }
+        first = __shfl(first, leader);
+        const uint32_t slot = first + __popcll(hm & ((1ull << lane) - 1));
+        if (slot < P.hnum) {
+          P.hits[slot] = h;
+          P.hit_photon[slot] = ph.idx;
+        }
+        if (P.xR == 1)
+          TOT = 0;
+        else
+          old = om;
+      }
+    }
+
+    if (TOT < XXX) {
+      TOT = 0;
+      live = false;
+    } else {
+      if (EXT && P.sum > 0 && fin > 0) {  // birefringence (gpu/pro.cu:757-836)
+        float ra, rb;
+        if (I == J) {
+          float4 a = T.az[I];
+          ra = fin * a.z, rb = fin * a.w;
+        } else {
+          const float y = z + n.z * fin / edh;
+          const float h = 0.5f - sign * (z - I);
+          const float g = 0.5f + sign * (y - J);
+          float4 aI = T.az[I], aJ = T.az[J];
+          ra = h * aI.z + g * aJ.z;
+          rb = h * aI.w + g * aJ.w;
+          for (int i = I + sign; i != J; i += sign) {
+            float4 a = T.az[i];
+            ra += a.z, rb += a.w;
+          }
+          const float f = edh / anz;
+          ra *= f, rb *= f;
+        }
+        if (ra < 0) ra = 0;
+        if (rb < 0) rb = 0;
+        const float dot = P.azx * n.x + P.azy * n.y;
+        const float sdt = sqrtf(1 - dot * dot);
+        const float cdt = fabsf(dot);
+        const float* B = P.bfr;
+        const float sra = sqrtf(ra);
+        float sx = sra * fmaxf(0.f, B[0] * fexp(-B[1] * fpow(atanf(B[3] * sdt), B[2])));
+        float sy = sra * fmaxf(0.f, B[4] * fexp(-B[5] * fpow(atanf(B[7] * sdt), B[6])));
+        float mx = rb * fmaxf(0.f, B[8] * atanf(B[11] * sdt * cdt) * fexp(-B[9] * sdt + B[10] * cdt));
+        float qx = P.azx - dot * n.x, qy = P.azy - dot * n.y, qz = -dot * n.z;
+        if (dot < 0) qx = -qx, qy = -qy, qz = -qz;
+        float qn = qx * qx + qy * qy + qz * qz;
+        if (qn > 0) {
+          float f = frsq(qn);
+          qx *= f, qy *= f, qz *= f;
+        } else
+          qx = 0, qy = 0, qz = 1;
+        const float px = n.y * qz - n.z * qy, py = n.z * qx - n.x * qz, pz = n.x * qy - n.y * qx;
+        float2 g2 = gauss2_rnd(s);
+        float dnx = sx * g2.x + mx;
+        float dny = sy * g2.y;
+        if (del <= hi) {  // curved path within the step
+          g2 = gauss2_rnd(s);
+          const float r3 = 0.57735026919f;
+          float vnx = dnx + sx * g2.x * r3;
+          float vny = dny + sy * g2.y * r3;
+          vnx /= 2, vny /= 2;
+          float den = vnx * vnx + vny * vny;
+          float dnz = -den;
+          den = 2 / (1 + den);
+          vnx *= den, vny *= den, dnz *= den;
+          float mxv = n.x + qx * vnx + px * vny + n.x * dnz;
+          float myv = n.y + qy * vnx + py * vny + n.y * dnz;
+          float mzv = n.z + qz * vnx + pz * vny + n.z * dnz;
+          normalize3(mxv, myv, mzv);
+          float dl = del - fin / 2;
+          r.x += dl * (mxv - n.x), r.y += dl * (myv - n.y), r.z += dl * (mzv - n.z);
+        }
+        dnx /= 2, dny /= 2;
+        float den = dnx * dnx + dny * dny;
+        float dnz = -den;
+        den = 2 / (1 + den);
+        dnx *= den, dny *= den, dnz *= den;
+        n.x += qx * dnx + px * dny + n.x * dnz;
+        n.y += qy * dnx + py * dny + n.y * dnz;
+        n.z += qz * dnx + pz * dny + n.z * dnz;
+        normalize3(n.x, n.y, n.z);
+      }
+      if (SCA < XXX) {  // scattering: HG with probability 1-sf, simplified Liu otherwise
+        SCA = 0;
+        float sf, g, gr;
+        if (EXT && hole)
+          sf = P.SF, g = P.G, gr = P.GR;
+        else
+          sf = P.sf, g = P.g, gr = P.gr;
+        float xi = s();
+        if (xi > sf) {
+          xi = (1 - xi) / (1 - sf);
+          xi = 2 * xi - 1;
+          if (g != 0) {
+            float ga = (1 - g * g) / (1 + g * xi);
+            xi = (ga * ga - sq(1 - g)) / (2 * g);
+          }
+        } else {
+          xi = fdiv(xi, sf);
+          xi = 2 * (1 - fpow(xi, gr));
+        }
+        xi = fminf(fmaxf(xi, 0.f), 2.f);
+        const float4 a = T.az[J];
+        const float k1 = a.x, k2 = a.y, okz = k1 * k2;
+        const bool frame = !(EXT && hole);
+        if (frame) {  // into the frame of anisotropic scattering
+          float n1 = (P.azx * n.x + P.azy * n.y) * k1;
+          float n2 = (-P.azy * n.x + P.azx * n.y) * k2;
+          n.x = n1 * P.azx - n2 * P.azy;
+          n.y = n1 * P.azy + n2 * P.azx;
+          n.z = fdiv(n.z, okz);
+          normalize3(n.x, n.y, n.z);
+        }
+        rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+        if (frame) {  // and back
+          float n1 = fdiv(P.azx * n.x + P.azy * n.y, k1);
+          float n2 = fdiv(-P.azy * n.x + P.azx * n.y, k2);
+          n.x = n1 * P.azx - n2 * P.azy;
+          n.y = n1 * P.azy + n2 * P.azx;
+          n.z *= okz;
+          normalize3(n.x, n.y, n.z);
+        }
+      }
+    }
+    ph.r = r, ph.n = n;
+    ph.TOT = TOT, ph.SCA = SCA;
+    ph.old = SCA == 0 ? -1 : old;  // the next step draws SCA and forgets the last DOM
+    }
+  }
+
+  P.rs[gid] = make_uint2(s.x, s.c);
+
+  // per-launch counters: one atomic per wave
+  unsigned long long v[4] = {n_steps, n_photons, n_killed, n_nonfinite};
+  for (int q = 0; q < 4; q++) {
+    unsigned long long t = v[q];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    v[q] = t;
+  }
+  if (lane == 0) {
+    if (v[0]) atomicAdd(&P.ctr->steps, v[0]);
+    if (v[1]) atomicAdd(&P.ctr->photons, v[1]);
+    if (v[2]) atomicAdd(&P.ctr->killed, v[2]);
+    if (v[3]) atomicAdd(&P.ctr->nonfinite, v[3]);
+  }
+}
+
+// ---------------------------------------------------------------- launcher
+
+typedef void (*kfun)(const KParams);
+
+static kfun pick(int tilt, bool ext) {
+  if (ext) {
+    if (tilt == 1) return propagate<1, true>;
+    if (tilt == 2) return propagate<2, true>;
+    return propagate<0, true>;
+  }
+  if (tilt == 1) return propagate<1, false>;
+  if (tilt == 2) return propagate<2, false>;
+  return propagate<0, false>;
+}
+
+hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream) {
+  const bool ext = P.sum > 0 || P.hr > 0 || P.fr != 0;
+  kfun f = pick(P.tmod, ext);
+  hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, P);
+  return hipGetLastError();
+}
+
+}  // namespace ppc
