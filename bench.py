@@ -1,0 +1,230 @@
+"""Benchmark: photons/s of the HIP propagation kernel on the 86-string cascade.
+
+Workload: the 86-string synthetic array (86 strings x 60 DOMs on a 125 m hex
+grid), 170 ice layers with 1-d tilt and 8 % absorption anisotropy, and a
+100 TeV EM cascade at 2000 m depth whose Poisson mean is 1e9 photons
+(ppc_amd.configs: c2).  One step = one launch that propagates all 1e9 photons
+of the cascade; inputs (tables, bunch) are resident on the device, and the
+hits stay in HBM (their count is read back).
+
+The JSON line also carries
+  roofline      algorithmic FP32 work of the propagation kernel per launch
+                (event counts from a counting launch x the per-event operation
+                counts of FLOPS below) over its measured duration, against the
+                MI355X FP32 vector peak;
+  cpu_baseline  the reference's own CPU program (oracle/_ref/ppc, single
+                thread per process) on the same ice, one process per host core,
+                on a bounded cascade sample; the oracle restatement
+                ("port") where the reference program is not built.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (AMD published)
+
+# Algorithmic FP32 operations of the reference algorithm per event, counted from
+# gpu/pro.cu (add/mul = 1, FMA = 2; divide, sqrt, rsqrt, log, exp, sin, cos = 1).
+FLOPS = {
+    "step": 69,       # tilt-free step: anisotropy, final layer-walk iteration, cell box, advance
+    "tilt1": 20,      # 1-d tilt interpolation, per step (pro.cu:169-191)
+    "tilt2": 50,      # 2-d tilt interpolation, per step (pro.cu:192-262)
+    "crossing": 10,   # one layer boundary in the layer walk (pro.cu:554-560)
+    "string": 17,     # one string cylinder test (pro.cu:601-608)
+    "dom": 30,        # one DOM ellipsoid test (pro.cu:631-651)
+    "scatter": 136,   # HG/SAM angle, anisotropic frame in/out, rotate, next SCA draw (pro.cu:838-886, 501)
+    "photon": 120,    # emission (source sampling, one rotate) and the TOT draw (pro.cu:334-477, 496)
+    "hit": 25,        # hit record: angles and impact point (pro.cu:731-750)
+}
+
+
+def gpu_run(args):
+    from ppc_amd import api, configs
+
+    cas = configs.c2_cascade()
+    det = api.Detector(configs.ensure("c2"), {"WFLA": None, "FWID": None, "DTMN": None})
+    n = configs.f2k_direction(cas["th"], cas["ph"])
+    photons = int(args.photons)
+
+    # event counts per photon from one counting launch (separate kernel build)
+    cal = api.Device(0, count_events=True)
+    cal.upload(det)
+    b, mean = det.particle_bunch(cas["x"], cas["y"], cas["z"], cas["t"], cas["E"], 0.0, 11, n, num=photons // 10)
+    c = cal.run([b])
+    cal.close()
+    per = {k: getattr(c, k) / c.photons for k in ("steps", "crossings", "strings", "doms", "scatters", "hits")}
+    fl_photon = (per["steps"] * (FLOPS["step"] + FLOPS["tilt1"]) + per["crossings"] * FLOPS["crossing"]
+                 + per["strings"] * FLOPS["string"] + per["doms"] * FLOPS["dom"]
+                 + per["scatters"] * FLOPS["scatter"] + FLOPS["photon"] + per["hits"] * FLOPS["hit"])
+
+    dev = api.Device(0)
+    dev.upload(det)
+    b.num = photons
+    for _ in range(args.warmup):
+        dev.run([b])
+    dev.wait()
+    ms = []
+    steps = hits = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.launch([b])
+        s = dev.wait()
+        ms.append(s.device_ms)
+        steps += s.steps
+        hits += s.hits
+        assert s.photons == photons
+    t1 = time.perf_counter()
+    dev.close()
+    wall = t1 - t0
+    kern_ms = sum(ms) / len(ms)
+    achieved = photons * fl_photon / (kern_ms * 1e-3) / 1e12
+    return {
+        "value": photons * args.steps / wall,
+        "ms_per_step": wall / args.steps * 1e3,
+        "kernel_ms": kern_ms,
+        "mean_photons": mean,
+        "steps_per_photon": steps / (photons * args.steps),
+        "hits_per_photon": hits / (photons * args.steps),
+        "flop_per_photon": fl_photon,
+        "events_per_photon": per,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32, 4), "traffic": None},
+        "cascade": cas,
+    }
+
+
+def cpu_reference(cores, seconds):
+    """the reference's CPU program, one process per core, cascade photons from stdin"""
+    from ppc_amd import configs
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ppc")
+    ice = configs.ensure("c2")
+    cas = configs.c2_cascade()
+    env = dict(os.environ, PPCTABLESDIR=ice)
+    for k in ("WFLA", "FWID"):
+        env.pop(k, None)
+
+    def run(E, k):
+        line = "TR 1 0 e %.3f %.3f %.3f %.4f %.4f 0 %.6g 0\n" % (cas["x"], cas["y"], cas["z"], cas["th"], cas["ph"], E)
+        return subprocess.Popen([exe, str(k)], env=env, stdin=subprocess.PIPE, stdout=subprocess.DEVNULL,
+                                stderr=subprocess.PIPE, text=True), line
+
+    # set-up time of one process (tables, 131072 multipliers) from an empty event
+    t0 = time.perf_counter()
+    p, line = run(1e-6, 0)
+    p.communicate(line, timeout=120)
+    init = time.perf_counter() - t0
+    # ~1e5 photons/s per core: size the cascade for `seconds` of propagation per process
+    E = 1e5 * 1e5 * seconds / 1e9
+    t0 = time.perf_counter()
+    procs = [run(E, k) for k in range(cores)]
+    for p, l in procs:
+        p.stdin.write(l)
+        p.stdin.close()
+    photons = 0
+    for p, _ in procs:
+        err = p.stderr.read()
+        p.wait(timeout=600)
+        photons += sum(int(m) for m in re.findall(r"photons: (\d+)", err))
+    wall = time.perf_counter() - t0 - init
+    return {"value": photons / wall, "unit": "photons/s", "cores": cores, "kind": "reference",
+            "sample": "%d processes x one %.0f GeV EM cascade (%.3g photons in total) on the 86-string ice, "
+                      "reference CPU program gpu/ppc.cu built with -O2 --fast-math -DDTMN; %.1f s after %.2f s "
+                      "set-up" % (cores, E, photons, wall, init)}
+
+
+def cpu_port(cores, seconds):
+    """the oracle restatement (C), one process per core, when the reference program is absent"""
+    code = r'''
+import ctypes, sys, time
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + "/tests")
+import numpy as np
+import oracle_util as U
+from ppc_amd import api, configs
+cas = configs.c2_cascade()
+D = api.Detector(configs.ensure("c2"), {"WFLA": None, "FWID": None})
+T = U.product_tables(D)
+n = configs.f2k_direction(cas["th"], cas["ph"])
+b = U.cascade_bunch(cas["x"], cas["y"], cas["z"], n[0], n[1], n[2], int(sys.argv[2]))
+k = int(sys.argv[3])
+s = U.Rng(362436069 + k, 1234567, int(D.array("rm", np.uint32)[k]))
+H = (U.Hit * 100000)()
+t0 = time.perf_counter()
+U.oracle().orc_run_xcpu(T.ptr, ctypes.byref(b), 1, b.num, 512, ctypes.byref(s), H, 100000, None)
+print(b.num, time.perf_counter() - t0)
+'''
+    num = int(1e5 * seconds)
+    procs = [subprocess.Popen([sys.executable, "-c", code, ROOT, str(num), str(k)], stdout=subprocess.PIPE,
+                              stderr=subprocess.DEVNULL, text=True) for k in range(cores)]
+    photons, wall = 0, 0.0
+    for p in procs:
+        out = p.communicate(timeout=900)[0].split()
+        photons += int(out[0])
+        wall = max(wall, float(out[1]))
+    return {"value": photons / wall, "unit": "photons/s", "cores": cores, "kind": "port",
+            "sample": "%d processes x %d photons of the 100 TeV cascade on the 86-string ice, oracle restatement "
+                      "(oracle/ppc_oracle.c)" % (cores, num)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=float, default=1e9)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the propagator runs on a single MI355X (one photon stream per lane, "
+              "no cross-GPU exchange); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+    g = gpu_run(args)
+    cpu = None
+    if not args.no_cpu:
+        cores = max(1, min(16, os.cpu_count() or 1))
+        if os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ppc")):
+            cpu = cpu_reference(cores, args.cpu_seconds)
+        else:
+            cpu = cpu_port(cores, args.cpu_seconds)
+    out = {
+        "metric": "photons/s on 1 GPU (86-string cascade) vs host CPU",
+        "value": g["value"],
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "86-string cascade: 86x60 DOMs, 170 layers, 1-d tilt, 8%% absorption anisotropy, "
+                               "100 TeV EM cascade, %.3g photons per step" % args.photons,
+                   "photons_per_step": int(args.photons), "kernel_ms": round(g["kernel_ms"], 3),
+                   "steps_per_photon": round(g["steps_per_photon"], 2),
+                   "hits_per_photon": g["hits_per_photon"], "flop_per_photon": round(g["flop_per_photon"], 1)},
+        "roofline": g["roofline"],
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["config"]["gpu_over_cpu"] = round(g["value"] / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

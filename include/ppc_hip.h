@@ -146,7 +146,7 @@ typedef struct {
 /* launch shape and capacities (the reference derives these from the device,
  * gpu/ppc.cu:141-186; 0 selects the library default) */
 typedef struct {
-  int32_t blocks;          /* workgroups; default: 2 per CU */
+  int32_t blocks;          /* workgroups; default: 1024/threads per CU (16 waves per CU) */
   int32_t threads;         /* lanes per workgroup (multiple of 64); default 256 */
   int32_t refill;          /* refill a wave once this many of its 64 lanes are idle; default 16 */
   int32_t deterministic;   /* 1: photon i always runs on lane i % lanes (the reference's DTMN) */
@@ -154,6 +154,7 @@ typedef struct {
   uint32_t max_hits;       /* hit buffer capacity; default 1<<24 */
   uint32_t rec_stride;     /* record the MWC state at emission of every rec_stride-th photon (0: off) */
   uint32_t rec_max;        /* capacity of the record buffer */
+  int32_t count_events;    /* 1: also count layer crossings, string and DOM tests, scatterings (slower) */
 } ppc_hip_config;
 
 /* per-launch results (what gpu/ppc.cu:255-283 reads back) */
@@ -166,6 +167,11 @@ typedef struct {
   uint64_t nonfinite; /* photons dropped because TOT/SCA became inf/nan */
   float device_ms;    /* kernel time between HIP events on the launch stream */
   int32_t lanes;      /* lanes of the launch */
+  /* event counts, filled when count_events is set */
+  uint64_t crossings; /* layer boundaries crossed in the layer walk */
+  uint64_t strings;   /* string (cylinder) intersection tests */
+  uint64_t doms;      /* DOM (ellipsoid) intersection tests */
+  uint64_t scatters;  /* scatterings */
 } ppc_hip_stats;
 
 /* replay record: MWC state of a lane when it starts emitting photon `photon` */

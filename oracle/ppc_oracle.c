@@ -171,6 +171,28 @@ void orc_rotate(float cs, float si, float n[4], orc_rng* s) {
   }
 }
 
+/* 1 - cos of the scattering angle: Henyey-Greenstein with probability 1-sf,
+ * simplified Liu otherwise (pro.cu:848-862) */
+float orc_scatter(float sf, float g, float gr, orc_rng* s) {
+  float xi = orc_xrnd(s);
+  if (xi > sf) { /* Henyey-Greenstein */
+    xi = (1 - xi) / (1 - sf);
+    xi = 2 * xi - 1;
+    if (g != 0) {
+      float ga = (1 - g * g) / (1 + g * xi);
+      xi = (ga * ga - sq(1 - g)) / (2 * g);
+    }
+  } else { /* simplified Liu */
+    xi /= sf;
+    xi = 2 * (1 - powf(xi, gr));
+  }
+  if (xi < 0)
+    xi = 0;
+  else if (xi > 2)
+    xi = 2;
+  return xi;
+}
+
 /* ------------------------------------------------------------------ tilt */
 
 static int findcol(const orc_tables* t, int x, int y) {
@@ -746,22 +768,7 @@ int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* h
         sf = t->SF, g = t->G, gr = t->GR;
       else
         sf = t->sf, g = t->g, gr = t->gr;
-      float xi = orc_xrnd(s);
-      if (xi > sf) { /* Henyey-Greenstein */
-        xi = (1 - xi) / (1 - sf);
-        xi = 2 * xi - 1;
-        if (g != 0) {
-          float ga = (1 - g * g) / (1 + g * xi);
-          xi = (ga * ga - sq(1 - g)) / (2 * g);
-        }
-      } else { /* simplified Liu */
-        xi /= sf;
-        xi = 2 * (1 - powf(xi, gr));
-      }
-      if (xi < 0)
-        xi = 0;
-      else if (xi > 2)
-        xi = 2;
+      float xi = orc_scatter(sf, g, gr, s);
       float k1 = t->az_k1[J], k2 = t->az_k2[J];
       float okz = k1 * k2;
       if (!hole) { /* into the frame of anisotropic scattering */

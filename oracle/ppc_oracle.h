@@ -9,6 +9,7 @@
  *   gpu/pro.cu:47-57    MWC draw                      orc_xrnd
  *   gpu/pro.cu:59-95    gamma / gaussian samplers     orc_mrnd, orc_grnd, orc_grnd2
  *   gpu/pro.cu:97-153   direction rotation            orc_rotate
+ *   gpu/pro.cu:848-862  scattering angle              orc_scatter
  *   gpu/pro.cu:164-264  1-D / 2-D tilt                orc_zshift
  *   gpu/pro.cu:334-477  photon emission               orc_emit
  *   gpu/pro.cu:482-889  transport step loop           orc_transport
@@ -142,6 +143,7 @@ float orc_xrnd(orc_rng* s);
 float orc_mrnd(float k, orc_rng* s);
 float orc_grnd(orc_rng* s);
 void orc_rotate(float cs, float si, float n[4], orc_rng* s);
+float orc_scatter(float sf, float g, float gr, orc_rng* s);
 float orc_zshift(const orc_tables* t, const float r[4], float* dz);
 
 void orc_emit(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, orc_rng* s, orc_pbuf* out);

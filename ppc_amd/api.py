@@ -49,13 +49,14 @@ RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<
 class Config(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32), ("refill", ctypes.c_int32),
                 ("deterministic", ctypes.c_int32), ("max_bunches", ctypes.c_uint32), ("max_hits", ctypes.c_uint32),
-                ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32)]
+                ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32), ("count_events", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("photons", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("hits", ctypes.c_uint64),
                 ("stored", ctypes.c_uint64), ("killed", ctypes.c_uint64), ("nonfinite", ctypes.c_uint64),
-                ("device_ms", ctypes.c_float), ("lanes", ctypes.c_int32)]
+                ("device_ms", ctypes.c_float), ("lanes", ctypes.c_int32), ("crossings", ctypes.c_uint64),
+                ("strings", ctypes.c_uint64), ("doms", ctypes.c_uint64), ("scatters", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -194,10 +195,10 @@ class Device:
     """One GPU context of libppc_hip.so."""
 
     def __init__(self, device=0, blocks=0, threads=0, refill=0, deterministic=False, max_bunches=0, max_hits=0,
-                 rec_stride=0, rec_max=0):
+                 rec_stride=0, rec_max=0, count_events=False):
         self.lib = hip()
         self.cfg = Config(blocks, threads, refill, 1 if deterministic else 0, max_bunches, max_hits, rec_stride,
-                          rec_max)
+                          rec_max, 1 if count_events else 0)
         self.ctx = ctypes.c_void_p()
         _check(self.lib.ppc_hip_open(device, ctypes.byref(self.cfg), ctypes.byref(self.ctx)))
         n = ctypes.c_int32()

@@ -212,6 +212,15 @@ def write_c2(path, seed=2, nrnd=131072):
     return path
 
 
+def f2k_direction(th, ph):
+    """propagation direction of an f2k track given its zenith/azimuth in degrees
+    (the source direction, gpu/f2k.cxx:567-569)"""
+    fcv = math.pi / 180
+    t = fcv * (180 - th)
+    p = fcv * (ph + 180 if ph < 180 else ph - 180)
+    return (math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t))
+
+
 def c2_cascade(seed=2):
     """The 100 TeV EM cascade of the 86-string workload: f2k TR line fields
     (x, y, z in the detector frame, zenith/azimuth in degrees, E in GeV)."""
@@ -224,7 +233,81 @@ def c2_cascade(seed=2):
     return dict(x=float(x), y=float(y), z=float(ZOFF - depth), th=float(th), ph=float(ph), E=1e5, t=0.0)
 
 
-_WRITERS = {"c1": write_c1, "c2": write_c2}
+# birefringence parameters of a published ice model (cfg.txt format, doc/index.rst:548-561)
+BFR12 = [0.076795, 544284.545166, 2.229494, 0.002624, 0.077381, 1547618.883526, 2.449589, 0.002505,
+         0.000995, 0.248264, 2.354436, 1.680717]
+
+
+def write_c2x(path, seed=3, nrnd=131072):
+    """The 86-string ice with every optional feature of the kernel switched on:
+    2-D tilt map (tilt.set/tilt.map), birefringence, hole ice, per-layer
+    scattering anisotropy and birefringence strength, old absorption
+    anisotropy, air bubbles and the dust/ice absorption split (A, B of
+    icemodel.par); plus RDE, high-voltage, cable-position and high-QE
+    acceptance files for the hit post-processing."""
+    os.makedirs(path, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    strings = _string_positions(86)
+    _write_geo(os.path.join(path, "geo-f2k"), strings)
+    n = 170
+    depths = 1105.0 + 10.0 * np.arange(n)
+    be = np.exp(rng.uniform(math.log(0.02), math.log(0.1), n))
+    ba = np.exp(rng.uniform(math.log(0.005), math.log(0.03), n))
+    td = rng.uniform(-10.0, 10.0, n)
+    k1 = rng.uniform(0.0, 0.08, n)
+    k2 = -k1 / 2
+    bfra = rng.uniform(0.8, 1.2, n)
+    bfrb = rng.uniform(0.8, 1.2, n)
+    with open(os.path.join(path, "icemodel.dat"), "w") as f:
+        for row in zip(depths, be, ba, td, k1, k2, bfra, bfrb):
+            f.write("%.2f %.8g %.8g %.5g %.5g %.5g %.5g %.5g\n" % row)
+    with open(os.path.join(path, "icemodel.par"), "w") as f:
+        f.write("0.898608505726 0.027638472617\n1.084106802940 0.014470303431\n")
+        f.write("6954.090332031250 973.426452636719\n6617.754394531250 71.282264709473\n")
+    with open(os.path.join(path, "icemodel.bbl"), "w") as f:
+        f.write("11.7 1350 1400\n")
+    lines = ["5", "%.9g" % (1e9 * 25.0 / (EM_YIELD * 1e5 * DPPM)), "0.35", "0.9", "", "130.0", "0.04", "-0.02", "",
+             "0.5", "0.03", "100", "0.35", "0.9", "", "0.6", "-0.3", "-0.3", "", "1.0", ""]
+    lines += ["%.9g" % v for v in BFR12]
+    with open(os.path.join(path, "cfg.txt"), "w") as f:
+        f.write("# ppc configuration file: follow strict order below\n" + "\n".join(lines) + "\n")
+    # 2-d tilt: 11 x 10 hexagonal grid along 9.3 / 129.3 deg with the SE and NW corners cut
+    mnum, mcut = (11, 10), (6, 5)
+    pts = [(i, j) for j in range(mnum[1]) for i in range(mnum[0]) if i < j + mcut[0] and j < i + mcut[1]]
+    with open(os.path.join(path, "tilt.set"), "w") as f:
+        f.write("9.3 -609.975 125.154 %d %d\n129.3 -549.188 125.402 %d %d\n" % (mnum[0], mcut[0], mnum[1], mcut[1]))
+    ph = rng.uniform(0, 2 * math.pi, len(pts))
+    amp = rng.uniform(3.0, 15.0, len(pts))
+    with open(os.path.join(path, "tilt.map"), "w") as f:
+        for dp in 1100.0 + 10.0 * np.arange(171):
+            vals = [amp[k] * math.sin(2 * math.pi * (dp - 1100.0) / 700.0 + ph[k]) for k in range(len(pts))]
+            f.write("%.2f " % dp + " ".join("%.4f" % v for v in vals) + "\n")
+    # post-processing inputs: RDE / high-QE, DOMs switched off, cable positions, high-QE acceptance
+    with open(os.path.join(path, "eff-f2k"), "w") as f:
+        for s in range(36, 45):
+            for d in range(20, 60, 3):
+                f.write("%d %d %.3f %d\n" % (s, d, rng.uniform(1.2, 1.4), int(s % 2)))
+    with open(os.path.join(path, "hvs-f2k"), "w") as f:
+        for s in range(1, 87):
+            for d in range(1, NDOM + 1):
+                f.write("%d %d %d\n" % (s, d, 0 if rng.uniform() < 0.02 else 1300))
+    with open(os.path.join(path, "dx.dat"), "w") as f:
+        for s in range(1, 87):
+            for d in range(1, NDOM + 1):
+                f.write("%d %d %.2f 10\n" % (s, d, rng.uniform(0.0, 360.0)))
+    with open(os.path.join(path, "wv.rde"), "w") as f:
+        for w in range(260, 700, 20):
+            f.write("%d %.4f\n" % (w, 1.0 + 0.35 * math.exp(-0.5 * ((w - 400.0) / 60.0) ** 2)))
+    with open(os.path.join(path, "as.dat"), "w") as f:
+        f.write("0.68\n0.34\n0.21\n0.08\n-0.03\n-0.01\n")
+    rnd = mwc_multipliers(nrnd)
+    with open(os.path.join(path, "rnd.txt"), "w") as f:
+        f.write("\n".join(str(int(a)) for a in rnd) + "\n")
+    _write_wv(path)
+    return path
+
+
+_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x}
 
 
 def ensure(name, nrnd=131072):

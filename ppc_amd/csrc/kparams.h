@@ -22,6 +22,7 @@ constexpr int kCTY = PPC_HIP_CTY;
 // global counters of one launch
 struct Counters {
   unsigned long long photons, steps, hits, killed, nonfinite;
+  unsigned long long crossings, strings, doms, scatters;
   unsigned int next;   // next photon to hand out (dynamic refill)
   unsigned int hidx;   // next free hit slot
   unsigned int nrec;   // records written

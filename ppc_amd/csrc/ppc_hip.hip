@@ -17,7 +17,8 @@
 #include "ppc_hip.h"
 
 namespace ppc {
-hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream,
+                            bool count);
 }
 
 using ppc::KParams;
@@ -129,7 +130,8 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
     delete c;
     return fail(PPC_HIP_E_ARG, "threads per workgroup must be a multiple of 64 up to 1024 (got %d)", k.threads);
   }
-  if (k.blocks <= 0) k.blocks = 2 * c->cus * (256 / k.threads > 0 ? 256 / k.threads : 1);
+  // 16 waves per CU (4 per SIMD: the kernel's register budget allows no more)
+  if (k.blocks <= 0) k.blocks = c->cus * (1024 / k.threads);
   if (k.refill <= 0) k.refill = 16;
   if (k.refill > 64) k.refill = 64;
   if (k.max_bunches == 0) k.max_bunches = 1u << 20;
@@ -394,7 +396,8 @@ int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, ui
   KParams P = c->P;
   P.nb = nb, P.num = num;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (num > 0) HIPCHK(ppc::launch_propagate(P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream));
+  if (num > 0)
+    HIPCHK(ppc::launch_propagate(P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, c->cfg.count_events != 0));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->launched = true;
   c->last_num = num;
@@ -419,6 +422,7 @@ int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {
     st->nonfinite = k.nonfinite;
     st->device_ms = ms;
     st->lanes = c->lanes;
+    st->crossings = k.crossings, st->strings = k.strings, st->doms = k.doms, st->scatters = k.scatters;
   }
   return PPC_HIP_OK;
 }

@@ -395,7 +395,7 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
 
 // ---------------------------------------------------------------- kernel
 
-template <int TILT, bool EXT>
+template <int TILT, bool EXT, bool COUNT>
 __global__ __launch_bounds__(1024) void propagate(const KParams P) {
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
@@ -427,6 +427,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
   bool done = false;  // no more photons to claim (wave-uniform)
   uint32_t next_static = gid;
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
+  uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;  // event counts (COUNT builds only)
 
   while (true) {
     // -------- refill idle lanes --------
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
           ais -= bs * del, aia -= ba * del;
           ahx = edh;
           J += sign;
+          if (COUNT) n_cross++;
         } else {
           sca += dsca, tot += dtot;
           break;
@@ -560,6 +562,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
             const uint32_t m = T.ls[k];
             k = (m & 0x8000u) ? 0x8000u : k + 1;
             const float4 la = T.sc[2 * (m & 0x7fffu)];
+            if (COUNT) n_str++;
             float b = 0, c = 0, dr;
             dr = la.x - r.x;
             b += n.x * dr, c += dr * dr;
@@ -586,6 +589,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
                 int n2 = sn + max(-1, min(smax, rd(-(zl - lb.z - la.w) * lb.x)));
                 for (int l = n1; l <= n2; l++) {
                   if (l == old) continue;
+                  if (COUNT) n_dom++;
                   const float4 dm = P.dom[l];
                   const float F = P.uniform_dom ? 1.f : P.domF[l];
                   float a = 0, bb = 0, cc = 0, d2;
@@ -785,6 +789,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
       }
       if (SCA < XXX) {  // scattering: HG with probability 1-sf, simplified Liu otherwise
         SCA = 0;
+        if (COUNT) n_scat++;
         float sf, g, gr;
         if (EXT && hole)
           sf = P.SF, g = P.G, gr = P.GR;
@@ -834,17 +839,18 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
   P.rs[gid] = make_uint2(s.x, s.c);
 
   // per-launch counters: one atomic per wave
-  unsigned long long v[4] = {n_steps, n_photons, n_killed, n_nonfinite};
-  for (int q = 0; q < 4; q++) {
+  constexpr int NC = COUNT ? 8 : 4;
+  unsigned long long v[8] = {n_steps, n_photons, n_killed, n_nonfinite, n_cross, n_str, n_dom, n_scat};
+  for (int q = 0; q < NC; q++) {
     unsigned long long t = v[q];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
     v[q] = t;
   }
   if (lane == 0) {
-    if (v[0]) atomicAdd(&P.ctr->steps, v[0]);
-    if (v[1]) atomicAdd(&P.ctr->photons, v[1]);
-    if (v[2]) atomicAdd(&P.ctr->killed, v[2]);
-    if (v[3]) atomicAdd(&P.ctr->nonfinite, v[3]);
+    unsigned long long* dst[8] = {&P.ctr->steps, &P.ctr->photons, &P.ctr->killed, &P.ctr->nonfinite,
+                                  &P.ctr->crossings, &P.ctr->strings, &P.ctr->doms, &P.ctr->scatters};
+    for (int q = 0; q < NC; q++)
+      if (v[q]) atomicAdd(dst[q], v[q]);
   }
 }
 
@@ -852,20 +858,22 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
 
 typedef void (*kfun)(const KParams);
 
+template <bool COUNT>
 static kfun pick(int tilt, bool ext) {
   if (ext) {
-    if (tilt == 1) return propagate<1, true>;
-    if (tilt == 2) return propagate<2, true>;
-    return propagate<0, true>;
+    if (tilt == 1) return propagate<1, true, COUNT>;
+    if (tilt == 2) return propagate<2, true, COUNT>;
+    return propagate<0, true, COUNT>;
   }
-  if (tilt == 1) return propagate<1, false>;
-  if (tilt == 2) return propagate<2, false>;
-  return propagate<0, false>;
+  if (tilt == 1) return propagate<1, false, COUNT>;
+  if (tilt == 2) return propagate<2, false, COUNT>;
+  return propagate<0, false, COUNT>;
 }
 
-hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream) {
+hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream,
+                            bool count) {
   const bool ext = P.sum > 0 || P.hr > 0 || P.fr != 0;
-  kfun f = pick(P.tmod, ext);
+  kfun f = count ? pick<true>(P.tmod, ext) : pick<false>(P.tmod, ext);
   hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, P);
   return hipGetLastError();
 }

@@ -1,0 +1,234 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own CPU build.
+
+Runs only where the reference tree and oracle/_ref are present (this container):
+    python tests/golden/make_golden.py
+
+Fixtures (data only: inputs and the reference's outputs on them):
+  mwc_rnd.json            first/last entries, count and SHA-256 of the reference's rnd.txt
+  table_<cfg>.txt         output of `ppc -` (ice table at the middle wavelength bin)
+  zshift_<cfg>.txt        output of `ppc =` on a grid of points (tilt)
+  stats_<case>.npz        raw hits of the reference kernel (gpu/pro.cu in its CPU form)
+                          on many photons: photons, per-DOM hit counts, hit times and DOMs
+  replay_<case>.npz       single photons from random MWC states: state before emission,
+                          state after transport, and every raw hit of each photon
+  cli_c1.npz              HIT lines of the reference command line `ppc 1 30 N 0` (C1 flasher)
+The synthetic ice directories are regenerated from their seeds by ppc_amd.configs.
+"""
+import ctypes
+import hashlib
+import json
+import pickle
+import tempfile
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import oracle_util as U  # noqa: E402
+from ppc_amd import configs  # noqa: E402
+
+REF_RND = "/root/reference/ice/spice_bfr-v2/rnd.txt"
+
+# name -> (ice dir, environment, source)
+CASES = {
+    "c1": ("c1", {"WFLA": "405", "FWID": "10"}, ("flasher", 1, 30)),
+    "c2": ("c2", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
+    "c2x": ("c2x", {}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
+}
+ENV_KEYS = ["WFLA", "FWID", "DTMN"]
+
+
+def _env(env):
+    e = dict(os.environ)
+    for k in ENV_KEYS:
+        e.pop(k, None)
+    e.update(env)
+    e["DTMN"] = "1"
+    return e
+
+
+def _bunch(T, src, num):
+    if src[0] == "flasher":
+        return U.flasher_bunch(T, src[1], src[2], num)
+    x, y, z, nx, ny = src[1:]
+    nz = -np.sqrt(1 - nx * nx - ny * ny)
+    return U.cascade_bunch(x, y, z, nx, ny, nz, num)
+
+
+def _fanout(kind, args_list, timeout=1800):
+    """run _<kind>_worker(args) in separate processes (the reference holds global state)"""
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    procs = []
+    for k, a in enumerate(args_list):
+        inp, out = os.path.join(tmp, "in%d.pkl" % k), os.path.join(tmp, "out%d.pkl" % k)
+        pickle.dump(a, open(inp, "wb"))
+        procs.append((subprocess.Popen([sys.executable, __file__, "_worker", kind, inp, out],
+                                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL), out))
+    res = []
+    for p, out in procs:
+        if p.wait(timeout=timeout) != 0:
+            raise RuntimeError("worker failed")
+        res.append(pickle.load(open(out, "rb")))
+    return res
+
+
+def _stats_worker(args):
+    case, k, batches = args
+    cfg, env, src = CASES[case]
+    for kk in ENV_KEYS:
+        os.environ.pop(kk, None)
+    T = U.ref_tables(configs.ensure(cfg), dict(env, DTMN="1"))
+    ref = U.reference()
+    pmax = ref.ref_pmax()
+    b = _bunch(T, src, pmax)
+    cap = 200000
+    H = (U.Hit * cap)()
+    a = ref.ref_multiplier(k)
+    x, c = 362436069 + 7919 * k, 1234567
+    hits = []
+    ox, oc = ctypes.c_uint32(), ctypes.c_uint32()
+    for _ in range(batches):
+        n = ref.ref_run(ctypes.byref(b), 1, pmax, 512, k, a, x, c, H, cap, ctypes.byref(ox), ctypes.byref(oc))
+        hits.append(U.hits_array(H, min(n, cap)))
+        x, c = ox.value, oc.value
+    return batches * pmax, np.concatenate(hits), int(T.get("gsize", np.int32)[0])
+
+
+STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
+
+
+def make_stats(case, procs=8, batches=8):
+    res = _fanout("stats", [(case, k, batches) for k in range(procs)])
+    photons = sum(r[0] for r in res)
+    hits = np.concatenate([r[1] for r in res])
+    gsize = res[0][2]
+    counts = np.bincount(hits["i"], minlength=gsize).astype(np.int64)
+    sub = np.sort(np.random.default_rng(7).permutation(len(hits))[:STATS_SAMPLE])
+    np.savez_compressed(os.path.join(HERE, "stats_%s.npz" % case), photons=np.int64(photons), counts=counts,
+                        nhits=np.int64(len(hits)), t=hits["t"][sub].astype(np.float32),
+                        dom=hits["i"][sub].astype(np.uint16), cth=np.cos(hits["pth"][sub]).astype(np.float32))
+    print("stats %s: %d photons, %d hits" % (case, photons, len(hits)))
+
+
+def _replay_worker(args):
+    case, p, n = args
+    cfg, env, src = CASES[case]
+    for kk in ENV_KEYS:
+        os.environ.pop(kk, None)
+    T = U.ref_tables(configs.ensure(cfg), dict(env, DTMN="1"))
+    ref = U.reference()
+    b = _bunch(T, src, 1)
+    states = replay_states(p, n)
+    cap = 1000
+    H = (U.Hit * cap)()
+    st0, st1, mult, hp, hh = [], [], [], [], []
+    for k in range(n):
+        x, c = int(states[k, 0]), int(states[k, 1])
+        a = ref.ref_multiplier(p * n + k)  # photon p*n+k runs on lane p*n+k of a deterministic launch
+        ox, oc = ctypes.c_uint32(), ctypes.c_uint32()
+        nh = ref.ref_run(ctypes.byref(b), 1, 1, 1, 0, a, x, c, H, cap, ctypes.byref(ox), ctypes.byref(oc))
+        st0.append((x, c)), st1.append((ox.value, oc.value)), mult.append(a)
+        h = U.hits_array(H, min(nh, cap))
+        hh.append(h)
+        hp.append(np.full(len(h), k, dtype=np.uint32))
+    return np.array(st0, dtype=np.uint32), np.array(st1, dtype=np.uint32), np.array(mult, dtype=np.uint32), \
+        np.concatenate(hp), np.concatenate(hh)
+
+
+def replay_states(p, n):
+    """MWC states (x, c) of the replayed photons of worker p, regenerated from the seed"""
+    rng = np.random.default_rng(1000 + p)
+    out = np.zeros((n, 2), dtype=np.uint32)
+    for k in range(n):
+        out[k] = int(rng.integers(0, 2 ** 32)), int(rng.integers(0, 2 ** 31))
+    return out
+
+
+def fold(st):
+    """32-bit digest of final MWC states (x, c)"""
+    x, c = st[:, 0].astype(np.uint32), st[:, 1].astype(np.uint32)
+    return x ^ ((c << np.uint32(16)) | (c >> np.uint32(16)))
+
+
+REPLAY_N, REPLAY_PROCS = 4096, 8
+
+
+def make_replay(case, n=REPLAY_N, procs=REPLAY_PROCS):
+    """Single photons from random states; photon k uses the multiplier of lane k.
+    Stored: the digest of each photon's final state and all its raw hits."""
+    res = _fanout("replay", [(case, p, n) for p in range(procs)])
+    st1 = np.concatenate([r[1] for r in res])
+    hp = np.concatenate([r[3] + p * n for p, r in enumerate(res)])
+    hh = np.concatenate([r[4] for r in res])
+    np.savez_compressed(os.path.join(HERE, "replay_%s.npz" % case), n=np.int64(n), procs=np.int64(procs),
+                        digest=fold(st1), hit_photon=hp.astype(np.uint32), hits=hh)
+    print("replay %s: %d photons, %d with hits, %d hits" % (case, len(st1), len(np.unique(hp)), len(hh)))
+
+
+def make_tables():
+    for cfg, env in [("c1", {"WFLA": "405"}), ("c2", {}), ("c2x", {})]:
+        out = subprocess.run([U.REF_EXE, "-"], env=_env(dict(env, PPCTABLESDIR=configs.ensure(cfg))),
+                             capture_output=True, text=True, check=True).stdout
+        open(os.path.join(HERE, "table_%s.txt" % cfg), "w").write(out)
+    rng = np.random.default_rng(5)
+    pts = np.column_stack([rng.uniform(-700, 700, 400), rng.uniform(-700, 700, 400), rng.uniform(-600, 600, 400)])
+    inp = "\n".join("%.3f %.3f %.3f" % tuple(p) for p in pts) + "\n"
+    for cfg in ["c2", "c2x"]:
+        out = subprocess.run([U.REF_EXE, "="], env=_env(dict(PPCTABLESDIR=configs.ensure(cfg), VTHK="1")), input=inp,
+                             capture_output=True, text=True, check=True).stdout
+        open(os.path.join(HERE, "zshift_%s.txt" % cfg), "w").write(out)
+
+
+def make_mwc():
+    vals = [int(v) for v in open(REF_RND).read().split()]
+    text = "\n".join(str(v) for v in vals) + "\n"
+    json.dump({"count": len(vals), "first": vals[:16], "last": vals[-4:],
+               "sha256": hashlib.sha256(text.encode()).hexdigest()},
+              open(os.path.join(HERE, "mwc_rnd.json"), "w"), indent=1)
+
+
+def make_cli(n=int(1.6e9)):
+    """accepted HIT lines of the reference command line on the C1 flasher"""
+    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure("c1")})
+    outs = []
+    procs = []
+    for k in range(8):
+        procs.append(subprocess.Popen([U.REF_EXE, "1", "30", "%d" % (n // 8), str(k)], env=env, stdout=subprocess.PIPE,
+                                      stderr=subprocess.PIPE, text=True))
+    photons = 0
+    for p in procs:
+        out, err = p.communicate()
+        outs.append(out)
+        photons += sum(int(l.split()[1]) for l in err.splitlines() if l.startswith("photons:"))
+    rows = [l.split() for o in outs for l in o.splitlines() if l.startswith("HIT")]
+    arr = np.array([[float(v) for v in r[1:]] for r in rows], dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "cli_c1.npz"), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
+                        dom=arr[:, 1].astype(np.int16), t=arr[:, 2].astype(np.float32),
+                        wv=arr[:, 3].astype(np.float32))
+    print("cli c1: %d photons, %d HIT lines" % (photons, len(rows)))
+
+
+if __name__ == "__main__":
+    if len(sys.argv) == 5 and sys.argv[1] == "_worker":
+        fn = {"stats": _stats_worker, "replay": _replay_worker}[sys.argv[2]]
+        pickle.dump(fn(pickle.load(open(sys.argv[3], "rb"))), open(sys.argv[4], "wb"))
+        sys.exit(0)
+    what = sys.argv[1:] or ["mwc", "tables", "replay", "stats", "cli"]
+    if "mwc" in what:
+        make_mwc()
+    if "tables" in what:
+        make_tables()
+    if "replay" in what:
+        for c in CASES:
+            make_replay(c)
+    if "stats" in what:
+        for c in CASES:
+            make_stats(c)
+    if "cli" in what:
+        make_cli()

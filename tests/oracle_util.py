@@ -80,6 +80,8 @@ def oracle():
         lib.orc_grnd.argtypes = [ctypes.POINTER(Rng)]
         lib.orc_rotate.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
                                    ctypes.POINTER(Rng)]
+        lib.orc_scatter.restype = ctypes.c_float
+        lib.orc_scatter.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.POINTER(Rng)]
         lib.orc_zshift.restype = ctypes.c_float
         lib.orc_zshift.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         lib.orc_photon.restype = ctypes.c_int
@@ -265,3 +267,18 @@ def to_orc_bunch(b):
     else:
         o.a, o.b, o.beta, o.tau = b.u.p.a, b.u.p.b, b.u.p.beta, b.u.p.tau
     return o
+
+
+def to_api_bunch(o):
+    """oracle Bunch -> ppc_amd.api.Bunch (the 64-byte C-ABI bunch)"""
+    from ppc_amd import api
+
+    b = api.Bunch()
+    b.r[:] = list(o.r)
+    b.n[:] = list(o.n)
+    b.q, b.num, b.type, b.f = o.q, o.num, o.type, o.f
+    if o.type > 0:
+        b.u.l.ka, b.u.l.up, b.u.l.fldr, b.u.l.fla, b.u.l.ofla = o.ka, o.up, o.fldr, o.fla, o.ofla
+    else:
+        b.u.p.a, b.u.p.b, b.u.p.beta, b.u.p.tau = o.a, o.b, o.beta, o.tau
+    return b
