@@ -73,7 +73,6 @@ def gpu_run(args):
     b.num = photons
     for _ in range(args.warmup):
         dev.run([b])
-    dev.wait()
     ms = []
     steps = hits = 0
     t0 = time.perf_counter()

@@ -61,6 +61,20 @@ float xppc_c_zshift(float x, float y, float z, float* dz);
 /* the host random-number state (seed of the MWC streams, gpu/ini.cxx:554) */
 uint32_t xppc_c_seed(void);
 
+/* the library-mode pipeline of the xppc interface (what llh drives, llh/ppc.h):
+ * choose + ini, flasher set-up, track segments, particles, flush, and the hitz
+ * map flattened into arrays (one entry per recorded photon, map order) */
+int xppc_c_start(int32_t device);     /* choose(device); ini() */
+int xppc_c_stop(void);                /* fin(); stop() */
+int xppc_c_flset(int32_t str, int32_t dom);
+int xppc_c_flone(uint64_t num);
+int xppc_c_sett(float nx, float ny, float nz, int32_t id, uint64_t sub, int32_t frame, double t0);
+int xppc_c_addp(float x, float y, float z, float t, float E, float dr, int32_t type, float scale);
+int xppc_c_eout(void);
+/* number of photon records in hitz; with non-NULL arrays, fills up to cap of them */
+uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* wv, uint64_t cap);
+int xppc_c_efin(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,36 @@ void ref_export(orc_tables* t) {
 
 uint32_t ref_multiplier(uint32_t k) { return z.rm[k]; }
 
+static void from_photon(const photon& p, orc_bunch& b) {
+  b.r[0] = p.r.x, b.r[1] = p.r.y, b.r[2] = p.r.z, b.r[3] = p.r.w;
+  b.n[0] = p.n.x, b.n[1] = p.n.y, b.n[2] = p.n.z, b.n[3] = p.n.w;
+  b.q = p.q, b.num = p.num, b.type = p.type, b.f = p.f;
+  if (p.type > 0) {
+    b.ka = p.ka, b.up = p.up, b.fldr = p.fldr, b.fla = p.fla, b.ofla = p.ofla;
+  } else {
+    b.a = p.a, b.b = p.b, b.beta = p.beta, b.tau = p.tau;
+  }
+}
+
+// The bunch addp() builds for a particle (gpu/f2k.cxx:484-504) with direction n,
+// without its Poisson draw: returns the mean photon count.
+double ref_particle(float x, float y, float zz, float t, float E, float dr, int type, float nx, float ny, float nz,
+                    orc_bunch* out) {
+  p.n.x = nx, p.n.y = ny, p.n.z = nz;
+  p.r.x = x, p.r.y = y, p.r.z = zz, p.r.w = t;
+  p.type = -type;
+  p.beta = 1, p.tau = 0;
+  double mean = yield(E, dr, type) / ovr;
+  from_photon(p, *out);
+  return mean;
+}
+
+// the flasher bunch flset() builds (gpu/f2k.cxx:630-759)
+void ref_flasher(int str, int dom, orc_bunch* out) {
+  flset(str, dom);
+  from_photon(p, *out);
+}
+
 static void to_photon(const orc_bunch& b, photon& p) {
   p.r.x = b.r[0], p.r.y = b.r[1], p.r.z = b.r[2], p.r.w = b.r[3];
   p.n.x = b.n[0], p.n.y = b.n[1], p.n.z = b.n[2], p.n.w = b.n[3];

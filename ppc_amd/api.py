@@ -115,6 +115,12 @@ def xppc():
         lib.xppc_c_zshift.argtypes = [ctypes.c_float] * 3 + [ctypes.POINTER(ctypes.c_float)]
         lib.xppc_c_seed.restype = ctypes.c_uint32
         lib.ppc_mwc_multipliers.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.xppc_c_flone.argtypes = [ctypes.c_uint64]
+        lib.xppc_c_sett.argtypes = [ctypes.c_float] * 3 + [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
+                                                            ctypes.c_double]
+        lib.xppc_c_addp.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int32, ctypes.c_float]
+        lib.xppc_c_hitz.restype = ctypes.c_uint64
+        lib.xppc_c_hitz.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_uint64]
         _xppc = lib
     return _xppc
 
@@ -178,6 +184,21 @@ class Detector:
 
     def flasher_photons(self, num):
         return int(self.lib.xppc_c_flasher_photons(num))
+
+    # ---- library-mode pipeline (xppc interface) ----
+    def start(self, device=0):
+        self.lib.xppc_c_start(device)
+
+    def stop(self):
+        self.lib.xppc_c_stop()
+
+    def hitz(self):
+        """the hitz map flattened: structured array (str, dom, pmt, t, wv)"""
+        n = int(self.lib.xppc_c_hitz(None, None, None, None, None, 0))
+        out = {k: np.zeros(n, dtype=d) for k, d in
+               [("str", np.int32), ("dom", np.int32), ("pmt", np.int32), ("t", np.float32), ("wv", np.float32)]}
+        self.lib.xppc_c_hitz(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("str", "dom", "pmt", "t", "wv")], n)
+        return out
 
     def zshift(self, x, y, z):
         dz = ctypes.c_float()

@@ -307,6 +307,48 @@ def write_c2x(path, seed=3, nrnd=131072):
     return path
 
 
+def c3_muon_event(seed=4, length=1200.0, step=1.0, E=1000.0):
+    """The muon workload on the 86-string ice: a 1 TeV muon on a straight 1.2 km
+    track through the array centre at a zenith in [0, 80] deg, as f2k track
+    records (gpu/f2k.cxx:555-583) -- 1 m "amu" segments with continuous light
+    and "brems" cascades at exponentially spaced points (mean 1 per 10 m) with
+    energies log-uniform in [1, 100] GeV.
+
+    Returns a list of dicts (name, x, y, z, th, ph, l, E, t) in the detector
+    frame; th/ph are the f2k zenith/azimuth of the track (source direction)."""
+    rng = np.random.default_rng(seed)
+    zen = math.radians(rng.uniform(0.0, 80.0))
+    azi = rng.uniform(0.0, 2 * math.pi)
+    # propagation direction: down-going from zenith (zen, azi)
+    d = np.array([-math.sin(zen) * math.cos(azi), -math.sin(zen) * math.sin(azi), -math.cos(zen)])
+    centre = np.array([rng.uniform(-50, 50), rng.uniform(-50, 50), ZOFF - 1950.0])
+    start = centre - d * length / 2
+    th, ph = math.degrees(zen), math.degrees(azi)
+    c = 0.299792458
+    recs = []
+    nseg = int(round(length / step))
+    for k in range(nseg):
+        p = start + d * (k * step)
+        recs.append(dict(name="amu", x=float(p[0]), y=float(p[1]), z=float(p[2]), th=th, ph=ph, l=step, E=E,
+                         t=k * step / c))
+    s = rng.exponential(10.0)
+    while s < length:
+        p = start + d * s
+        e = float(math.exp(rng.uniform(math.log(1.0), math.log(100.0))))
+        recs.append(dict(name="brems", x=float(p[0]), y=float(p[1]), z=float(p[2]), th=th, ph=ph, l=0.0, E=e, t=s / c))
+        s += rng.exponential(10.0)
+    return recs
+
+
+def f2k_lines(recs):
+    """the records as f2k TR lines"""
+    return ["TR 1 0 %s %.4f %.4f %.4f %.5f %.5f %.4f %.6g %.4f" % (r["name"], r["x"], r["y"], r["z"], r["th"], r["ph"],
+                                                                  r["l"], r["E"], r["t"]) for r in recs]
+
+
+F2K_TYPE = {"amu": 0, "brems": 11, "e": 11, "delta": 11, "epair": 11, "e-": 12, "e+": 13, "hadr": 101, "munu": 101}
+
+
 _WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x}
 
 

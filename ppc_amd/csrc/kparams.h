@@ -47,6 +47,8 @@ struct KParams {
   int mnum0, mnum1, mcut0, mcut1;
   int nsc, gsize;
   int uniform_dom;  // every DOM has R = OMR and F = 1
+  // scattering-function constants: 1/(1-sf), 1-g^2, (1-g)^2, 1/(2g), 1/sf; [0] bulk ice, [1] hole ice
+  float hg_isf1[2], hg_a[2], hg_b[2], hg_i2g[2], hg_isf[2];
 
   // ---- LDS image (byte offsets into it) ----
   const uint4* image;
@@ -75,6 +77,7 @@ struct KParams {
   ppc_hip_record* rec;
   uint32_t rec_stride, rec_max;
   int refill, deterministic;
+  uint32_t chunk;  // photons claimed per atomic on the global counter
   uint32_t lanes;
 };
 

@@ -17,8 +17,8 @@
 #include "ppc_hip.h"
 
 namespace ppc {
-hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream,
-                            bool count);
+hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int threads, size_t lds_bytes,
+                            hipStream_t stream, bool count);
 }
 
 using ppc::KParams;
@@ -75,6 +75,8 @@ struct ppc_hip_ctx {
   ppc_hip_bunch* h_pz = nullptr;
   uint32_t* h_pstart = nullptr;
   ppc::Counters* h_ctr = nullptr;
+  KParams* h_P = nullptr;  // pinned staging of the launch parameters
+  KParams* d_P = nullptr;  // the kernel reads its parameters from here
 
   // tables
   uint4* d_image = nullptr;
@@ -157,6 +159,8 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(hipHostMalloc((void**)&c->h_pz, sizeof(ppc_hip_bunch) * k.max_bunches, hipHostMallocDefault));
   chk(hipHostMalloc((void**)&c->h_pstart, sizeof(uint32_t) * (k.max_bunches + 1), hipHostMallocDefault));
   chk(hipHostMalloc((void**)&c->h_ctr, sizeof(ppc::Counters), hipHostMallocDefault));
+  chk(hipHostMalloc((void**)&c->h_P, sizeof(KParams), hipHostMallocDefault));
+  chk(dalloc(&c->d_P, 1));
   if (e != hipSuccess) {
     int code = e == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE;
     ppc_hip_close(c);
@@ -172,9 +176,9 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF})
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr})
+  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr, (void*)c->h_P})
     if (p) (void)hipHostFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -251,6 +255,16 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.mmin0 = t->mmin[0], P.mmin1 = t->mmin[1], P.mstp0 = t->mstp[0], P.mstp1 = t->mstp[1];
   P.mnum0 = t->mnum[0], P.mnum1 = t->mnum[1], P.mcut0 = t->mcut[0], P.mcut1 = t->mcut[1];
   P.nsc = t->nsc, P.gsize = t->gsize;
+  {
+    const float sfv[2] = {t->sf, t->SF}, gv[2] = {t->g, t->G};
+    for (int k = 0; k < 2; k++) {
+      P.hg_isf1[k] = 1.f / (1.f - sfv[k]);
+      P.hg_a[k] = 1.f - gv[k] * gv[k];
+      P.hg_b[k] = (1.f - gv[k]) * (1.f - gv[k]);
+      P.hg_i2g[k] = 1.f / (2.f * gv[k]);
+      P.hg_isf[k] = 1.f / sfv[k];
+    }
+  }
 
   // ---- LDS image ----
   std::vector<uint8_t> img;
@@ -370,6 +384,9 @@ int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, ui
   if (!c->uploaded) return fail(PPC_HIP_E_STATE, "launch before upload");
   if (nb > c->cfg.max_bunches) return fail(PPC_HIP_E_CAPACITY, "%u bunches > capacity %u", nb, c->cfg.max_bunches);
   if (num > 0 && (nb == 0 || !bunches)) return fail(PPC_HIP_E_ARG, "photons without bunches");
+  // the photon counter runs past num by at most one chunk (<= 1024) per wave
+  if ((uint64_t)num + (uint64_t)(c->lanes / 64 + 1) * 1024 > 0xffffffffull)
+    return fail(PPC_HIP_E_CAPACITY, "%u photons in one launch is too many", num);
   HIPCHK(hipSetDevice(c->device));
   // the staging buffers may still feed the previous copy
   if (c->copy_pending) HIPCHK(hipEventSynchronize(c->ev_copy));
@@ -390,14 +407,22 @@ int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, ui
     HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
   }
+  *c->h_P = c->P;
+  c->h_P->nb = nb, c->h_P->num = num;
+  {  // claim chunks of photons per wave: large enough to keep the global counter cold, small
+     // enough that the last chunks of a launch still spread over all waves
+    const uint32_t waves = (uint32_t)c->lanes / 64;
+    uint64_t ch = num / ((uint64_t)waves * 16 + 1);
+    c->h_P->chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ch, 64), 1024);
+  }
+  HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev_copy, c->stream));
   c->copy_pending = true;
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(ppc::Counters), c->stream));
-  KParams P = c->P;
-  P.nb = nb, P.num = num;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (num > 0)
-    HIPCHK(ppc::launch_propagate(P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, c->cfg.count_events != 0));
+    HIPCHK(ppc::launch_propagate(*c->h_P, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream,
+                                 c->cfg.count_events != 0));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->launched = true;
   c->last_num = num;

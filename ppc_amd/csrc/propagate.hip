@@ -395,8 +395,12 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
 
 // ---------------------------------------------------------------- kernel
 
+// The parameter block is read through a pointer into device memory: uniform
+// loads become scalar loads the compiler can repeat, where a by-value argument
+// of this size would be held in SGPRs and spilled into VGPR lanes.
 template <int TILT, bool EXT, bool COUNT>
-__global__ __launch_bounds__(1024) void propagate(const KParams P) {
+__global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp) {
+  const KParams& P = *Pp;
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
   __syncthreads();
@@ -426,6 +430,10 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
   bool live = false;
   bool done = false;  // no more photons to claim (wave-uniform)
   uint32_t next_static = gid;
+  // photons this wave has claimed but not started: [res_next, res_end) (wave-uniform);
+  // one atomic claims a chunk, so the global counter sees one request per chunk
+  uint32_t res_next = 0, res_end = 0;
+  const uint32_t chunk = P.chunk;
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;  // event counts (COUNT builds only)
 
@@ -440,13 +448,27 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
           if (!live) i = next_static, next_static += P.lanes;
           done = !__any(next_static < P.num);
         } else {
-          uint32_t first = 0;
-          const int leader = __ffsll((unsigned long long)idle) - 1;
-          if ((int)lane == leader) first = atomicAdd(&P.ctr->next, (uint32_t)nidle);
-          first = __shfl(first, leader);
           const uint32_t rank = __popcll(idle & ((1ull << lane) - 1));
-          if (!live) i = first + rank;
-          if ((unsigned long long)first + nidle >= P.num) done = true;
+          const uint32_t avail = res_end - res_next;
+          if (avail >= (uint32_t)nidle) {
+            if (!live) i = res_next + rank;
+            res_next += nidle;
+          } else {  // take the rest of the reservoir, then a new chunk from the global counter
+            uint32_t first = 0;
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            if ((int)lane == leader) first = atomicAdd(&P.ctr->next, chunk);
+            first = __builtin_amdgcn_readfirstlane(__shfl(first, leader));
+            if (!live) i = rank < avail ? res_next + rank : first + (rank - avail);
+            const uint32_t used = nidle - avail;
+            if (first >= P.num) {  // the launch is exhausted: the reservoir was the last work
+              res_next = res_end = 0;
+              done = true;
+            } else {
+              res_next = first + used;
+              res_end = (unsigned long long)first + chunk > P.num ? P.num : first + chunk;
+              if (res_next > res_end) res_next = res_end;
+            }
+          }
         }
         if (!live && i < P.num) {
           if (P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
@@ -790,21 +812,18 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
       if (SCA < XXX) {  // scattering: HG with probability 1-sf, simplified Liu otherwise
         SCA = 0;
         if (COUNT) n_scat++;
-        float sf, g, gr;
-        if (EXT && hole)
-          sf = P.SF, g = P.G, gr = P.GR;
-        else
-          sf = P.sf, g = P.g, gr = P.gr;
+        const int hk = (EXT && hole) ? 1 : 0;
+        const float sf = hk ? P.SF : P.sf, g = hk ? P.G : P.g, gr = hk ? P.GR : P.gr;
         float xi = s();
-        if (xi > sf) {
-          xi = (1 - xi) / (1 - sf);
+        if (xi > sf) {  // divisions by constants use the reciprocals prepared on the host
+          xi = (1 - xi) * P.hg_isf1[hk];
           xi = 2 * xi - 1;
           if (g != 0) {
-            float ga = (1 - g * g) / (1 + g * xi);
-            xi = (ga * ga - sq(1 - g)) / (2 * g);
+            float ga = P.hg_a[hk] * frcp(1 + g * xi);
+            xi = (ga * ga - P.hg_b[hk]) * P.hg_i2g[hk];
           }
         } else {
-          xi = fdiv(xi, sf);
+          xi *= P.hg_isf[hk];
           xi = 2 * (1 - fpow(xi, gr));
         }
         xi = fminf(fmaxf(xi, 0.f), 2.f);
@@ -856,7 +875,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams P) {
 
 // ---------------------------------------------------------------- launcher
 
-typedef void (*kfun)(const KParams);
+typedef void (*kfun)(const KParams*);
 
 template <bool COUNT>
 static kfun pick(int tilt, bool ext) {
@@ -870,11 +889,11 @@ static kfun pick(int tilt, bool ext) {
   return propagate<0, false, COUNT>;
 }
 
-hipError_t launch_propagate(const KParams& P, int blocks, int threads, size_t lds_bytes, hipStream_t stream,
-                            bool count) {
+hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int threads, size_t lds_bytes,
+                            hipStream_t stream, bool count) {
   const bool ext = P.sum > 0 || P.hr > 0 || P.fr != 0;
   kfun f = count ? pick<true>(P.tmod, ext) : pick<false>(P.tmod, ext);
-  hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, P);
+  hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, dP);
   return hipGetLastError();
 }
 

@@ -134,4 +134,60 @@ float xppc_c_zshift(float x, float y, float z, float* dz) {
 
 uint32_t xppc_c_seed(void) { return sv; }
 
+int xppc_c_start(int32_t device) {
+  if (!g_init) return -1;
+  choose(device);
+  ini();
+  return 0;
+}
+
+int xppc_c_stop(void) {
+  fin();
+  stop();
+  return 0;
+}
+
+int xppc_c_flset(int32_t str, int32_t dom) {
+  flset(str, dom);
+  return 0;
+}
+
+int xppc_c_flone(uint64_t num) {
+  flone(num);
+  return 0;
+}
+
+int xppc_c_sett(float nx, float ny, float nz, int32_t id, uint64_t sub, int32_t frame, double t0) {
+  sett(nx, ny, nz, std::make_pair((int)id, (unsigned long long)sub), frame, t0);
+  return 0;
+}
+
+int xppc_c_addp(float x, float y, float z, float t, float E, float dr, int32_t type, float scale) {
+  addp(x, y, z, t, E, dr, type, scale);
+  return 0;
+}
+
+int xppc_c_eout(void) {
+  eout();
+  return 0;
+}
+
+uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* wv, uint64_t cap) {
+  uint64_t n = 0;
+  for (const auto& e : hitz)
+    for (const pout& po : e.second) {
+      if (str && n < cap) {
+        str[n] = e.first.omkey.str, dom[n] = e.first.omkey.dom, pmt[n] = e.first.pmt;
+        t[n] = po.r[3], wv[n] = po.n[3];
+      }
+      n++;
+    }
+  return n;
+}
+
+int xppc_c_efin(void) {
+  efin();
+  return 0;
+}
+
 }  // extern "C"

@@ -23,6 +23,18 @@ using std::string;
 
 namespace xppc {
 
+// float arguments select the float overloads, as in the reference (gpu/ppc.cu:26)
+using std::acos;
+using std::atan2;
+using std::cos;
+using std::exp;
+using std::floor;
+using std::log;
+using std::pow;
+using std::sin;
+using std::sqrt;
+using std::tan;
+
 bool cli_mode = false;
 outz hitz;
 
@@ -76,11 +88,13 @@ unsigned int flnb = 0, flne = 0, flnd = 0;
 ihit tmph;
 float trz = 0, crz = 0;
 
-void pop_segment() {
+void pop_segment() {  // segments without an entry (no sett() call) keep the previous track id
   if (cli_mode) {
+    if (flnz_cli.empty()) return;
     std::cout << flnz_cli.front() << "\n";
     flnz_cli.pop_front();
   } else {
+    if (flnz_lib.empty()) return;
     tmph.track = flnz_lib.front();
     flnz_lib.pop_front();
   }

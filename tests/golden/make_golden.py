@@ -40,7 +40,10 @@ CASES = {
     "c1": ("c1", {"WFLA": "405", "FWID": "10"}, ("flasher", 1, 30)),
     "c2": ("c2", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
     "c2x": ("c2x", {}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
+    "c3": ("c2", {}, ("muon",)),
 }
+# single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
+C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
 ENV_KEYS = ["WFLA", "FWID", "DTMN"]
 
 
@@ -53,9 +56,20 @@ def _env(env):
     return e
 
 
-def _bunch(T, src, num):
+def _bunch(T, src, num, det=None):
+    """the single bunch of a case; for the muon case the segment bunch is made by
+    the product's front-end (det) or, here, by the reference's own addp()"""
     if src[0] == "flasher":
         return U.flasher_bunch(T, src[1], src[2], num)
+    if src[0] == "muon":
+        s = C3_SEGMENT
+        n = np.array(s["n"]) / np.linalg.norm(s["n"])
+        if det is not None:
+            b, _ = det.particle_bunch(s["x"], s["y"], s["z"], 0.0, s["E"], s["dr"], 0, n, num=num)
+            return U.to_orc_bunch(b)
+        b, _ = U.ref_particle(s["x"], s["y"], s["z"], 0.0, s["E"], s["dr"], 0, n)
+        b.num = num
+        return b
     x, y, z, nx, ny = src[1:]
     nz = -np.sqrt(1 - nx * nx - ny * ny)
     return U.cascade_bunch(x, y, z, nx, ny, nz, num)
@@ -78,12 +92,59 @@ def _fanout(kind, args_list, timeout=1800):
     return res
 
 
+def muon_event_bunches(make, procs=1, part=0):
+    """bunches of the muon event (records part, part+procs, ...): make(x,y,z,t,E,dr,type,n) -> (bunch, mean);
+    each bunch gets the rounded mean photon count and its record index as segment"""
+    out = []
+    for k, r in enumerate(configs.c3_muon_event()):
+        if k % procs != part:
+            continue
+        n = configs.f2k_direction(r["th"], r["ph"])
+        b, mean = make(r["x"], r["y"], r["z"], r["t"], r["E"], r["l"], configs.F2K_TYPE[r["name"]], n)
+        b.num = int(round(mean))
+        b.q = k
+        out.append(b)
+    return out
+
+
+def _muon_stats(k, procs, T):
+    ref = U.reference()
+    pmax = ref.ref_pmax()
+    bunches = muon_event_bunches(U.ref_particle, procs, k)
+    cap = 400000
+    H = (U.Hit * cap)()
+    a = ref.ref_multiplier(k)
+    x, c = 362436069 + 7919 * k, 1234567
+    ox, oc = ctypes.c_uint32(), ctypes.c_uint32()
+    hits, photons, i = [], 0, 0
+    while i < len(bunches):
+        grp, tot = [], 0
+        while i < len(bunches) and tot + bunches[i].num <= pmax:
+            grp.append(bunches[i])
+            tot += bunches[i].num
+            i += 1
+        if not grp:  # a bunch larger than one launch: split it
+            b = bunches[i]
+            part = U.Bunch.from_buffer_copy(b)
+            part.num = pmax
+            b.num -= pmax
+            grp, tot = [part], pmax
+        arr = (U.Bunch * len(grp))(*grp)
+        n = ref.ref_run(arr, len(grp), tot, 512, k, a, x, c, H, cap, ctypes.byref(ox), ctypes.byref(oc))
+        hits.append(U.hits_array(H, min(n, cap)))
+        x, c = ox.value, oc.value
+        photons += tot
+    return photons, np.concatenate(hits), int(T.get("gsize", np.int32)[0])
+
+
 def _stats_worker(args):
-    case, k, batches = args
+    case, k, batches = args[:3]
     cfg, env, src = CASES[case]
     for kk in ENV_KEYS:
         os.environ.pop(kk, None)
     T = U.ref_tables(configs.ensure(cfg), dict(env, DTMN="1"))
+    if src[0] == "muon":
+        return _muon_stats(k, args[3], T)
     ref = U.reference()
     pmax = ref.ref_pmax()
     b = _bunch(T, src, pmax)
@@ -104,7 +165,7 @@ STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 
 def make_stats(case, procs=8, batches=8):
-    res = _fanout("stats", [(case, k, batches) for k in range(procs)])
+    res = _fanout("stats", [(case, k, batches, procs) for k in range(procs)])
     photons = sum(r[0] for r in res)
     hits = np.concatenate([r[1] for r in res])
     gsize = res[0][2]
@@ -230,5 +291,10 @@ if __name__ == "__main__":
     if "stats" in what:
         for c in CASES:
             make_stats(c)
+    for c in CASES:
+        if "stats_" + c in what:
+            make_stats(c)
+        if "replay_" + c in what:
+            make_replay(c)
     if "cli" in what:
         make_cli()

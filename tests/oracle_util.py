@@ -282,3 +282,29 @@ def to_api_bunch(o):
     else:
         b.u.p.a, b.u.p.b, b.u.p.beta, b.u.p.tau = o.a, o.b, o.beta, o.tau
     return b
+
+
+def ref_particle(x, y, z, t, E, dr, ptype, n):
+    """the reference's addp() bunch (without its Poisson draw) and its mean photon count"""
+    lib = reference()
+    lib.ref_particle.restype = ctypes.c_double
+    lib.ref_particle.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int] + [ctypes.c_float] * 3 + [ctypes.POINTER(Bunch)]
+    b = Bunch()
+    mean = lib.ref_particle(x, y, z, t, E, dr, ptype, n[0], n[1], n[2], ctypes.byref(b))
+    return b, mean
+
+
+def ref_flasher(string, dom):
+    lib = reference()
+    lib.ref_flasher.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Bunch)]
+    b = Bunch()
+    lib.ref_flasher(string, dom, ctypes.byref(b))
+    return b
+
+
+def bunch_fields(b):
+    """comparable tuple of an oracle Bunch (the fields the kernel reads for its type)"""
+    base = tuple(np.float32(v) for v in list(b.r) + list(b.n)) + (b.q, b.type, np.float32(b.f))
+    if b.type > 0:
+        return base + (np.float32(b.ka), np.float32(b.up), np.float32(b.fldr), b.fla, b.ofla)
+    return base + (np.float32(b.a), np.float32(b.b), np.float32(b.beta), np.float32(b.tau))

@@ -1,0 +1,90 @@
+"""The product's front-end (ppc_amd/host/frontend.cpp) against the reference's
+own: the bunches addp() and flset() build (gpu/f2k.cxx:17-167, 484-504,
+630-759) for every particle type and for flashers under the beam-shaping
+environment variables, compared field by field with the reference's CPU build."""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_util as U
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+_CODE = r'''
+import sys, math
+import numpy as np
+sys.path.insert(0, sys.argv[2])
+import oracle_util as U
+from ppc_amd import api, configs
+T = U.ref_tables(sys.argv[1])
+D = api.Detector(sys.argv[1])
+bad = 0
+rng = np.random.default_rng(1)
+for ptype in [0, 1, 11, 12, 13, 14]:
+    for k in range(20):
+        x, y, z, t = rng.uniform(-300, 300, 4)
+        E = float(np.exp(rng.uniform(0, 12)))
+        dr = float(rng.choice([0.0, 1.0, 7.5])) if ptype < 10 else float(rng.choice([0.0, 3.0]))
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        rb, rmean = U.ref_particle(x, y, z, t, E, dr, ptype, n)
+        pb, pmean = D.particle_bunch(x, y, z, t, E, dr, ptype, n, num=0)
+        ob = U.to_orc_bunch(pb)
+        # same fields up to 2 ulp: the two builds may contract the yield arithmetic differently
+        fr, fo = np.array(U.bunch_fields(rb), dtype=np.float64), np.array(U.bunch_fields(ob), dtype=np.float64)
+        if not np.allclose(fr, fo, rtol=2.5e-7, atol=0) or abs(rmean - pmean) > 1e-6 * max(1.0, abs(rmean)):
+            bad += 1
+            print("MISMATCH", ptype, U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
+for s, d in [(1, 30), (2, 10), (0, 1), (0, 2), (-3, 5)]:
+    rb = U.ref_flasher(s, d)
+    pb = U.to_orc_bunch(D.flasher_bunch(s, d))
+    rb.q = pb.q = 0
+    if U.bunch_fields(rb) != U.bunch_fields(pb):
+        bad += 1
+        print("MISMATCH flasher", s, d, U.bunch_fields(rb), U.bunch_fields(pb))
+print("BAD", bad)
+'''
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("env", [{}, {"FWID": "10"}, {"FWID": "-1", "FLDR": "60"}, {"FWID": "1090", "FLDR": "30",
+                                                                                       "FTLT": "0.1 0.2 0.97"},
+                                 {"ELEV": "20", "FZCR": "3", "OFLA": "0", "FWID": "-2"}])
+def test_bunches_match_reference(env):
+    if not U.ref_available():
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    from ppc_amd import configs
+
+    e = dict(os.environ)
+    for k in ("WFLA", "FWID", "FLDR", "FTLT", "ELEV", "FZCR", "OFLA"):
+        e.pop(k, None)
+    e.update(env)
+    e["DTMN"] = "1"
+    e["PYTHONPATH"] = ROOT + os.pathsep + HERE
+    r = subprocess.run([sys.executable, "-c", _CODE, configs.ensure("c2"), HERE], env=e, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "BAD 0" in r.stdout, r.stdout[-3000:]
+
+
+def test_muon_event_records():
+    """the muon workload: 1200 segments of 1 m along one straight track, cascades
+    with mean spacing 10 m and log-uniform energies in [1, 100] GeV"""
+    from ppc_amd import configs
+
+    recs = configs.c3_muon_event()
+    seg = [r for r in recs if r["name"] == "amu"]
+    cas = [r for r in recs if r["name"] == "brems"]
+    assert len(seg) == 1200
+    assert 80 < len(cas) < 160
+    assert all(1.0 <= r["E"] <= 100.0 for r in cas)
+    p0 = np.array([seg[0][k] for k in "xyz"])
+    p1 = np.array([seg[-1][k] for k in "xyz"])
+    assert abs(np.linalg.norm(p1 - p0) - 1199.0) < 1e-3
+    d = np.array(configs.f2k_direction(seg[0]["th"], seg[0]["ph"]))
+    assert np.allclose((p1 - p0) / np.linalg.norm(p1 - p0), d, atol=1e-6)
+    assert 0 <= seg[0]["th"] <= 80

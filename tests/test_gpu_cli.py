@@ -1,0 +1,98 @@
+"""The drop-in paths on the GPU against the reference's own command line.
+
+The reference's CPU program ran the C1 flasher (`ppc 1 30 N dev`, WFLA=405,
+FWID=10) and its HIT lines are in tests/golden/cli_c1.npz.  Here the same
+command of this package (bin/ppc, the HIP kernel behind the xppc front-end
+and print()'s acceptance) and the library pipeline (initialize, choose, ini,
+flset, sett, flone, eout -> hitz, the calls llh makes) must reproduce the
+HIT rate per propagated photon (Poisson, p > 0.01), the distribution over
+DOMs (two-sample chi-square, p > 0.01) and the arrival-time and wavelength
+distributions (two-sample KS, p > 0.01).
+"""
+import math
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from test_gpu_golden import _two_sample_chi2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def _env():
+    from ppc_amd import configs
+
+    e = dict(os.environ)
+    e.update({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure("c1"), "DTMN": "1"})
+    return e
+
+
+def _compare(photons, strs, doms, t, wv):
+    gold = np.load(os.path.join(HERE, "golden", "cli_c1.npz"))
+    Nr, nr = int(gold["photons"]), len(gold["t"])
+    z = (len(t) / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + len(t) / photons ** 2)
+    p_rate = 2 * stats.norm.sf(abs(z))
+    key = lambda s, d: np.asarray(s, np.int64) * 100 + np.asarray(d, np.int64)
+    kg, kp = key(gold["str"], gold["dom"]), key(strs, doms)
+    allk = np.union1d(kg, kp)
+    cg = np.array([np.sum(kg == k) for k in allk])
+    cp = np.array([np.sum(kp == k) for k in allk])
+    chi2, dof, p_dom = _two_sample_chi2(cp, cg)
+    p_t = stats.ks_2samp(t, gold["t"]).pvalue
+    p_w = stats.ks_2samp(wv, gold["wv"]).pvalue
+    print("HIT rate %.4e vs %.4e (p=%.3f), DOM chi2 %.1f/%d (p=%.3f), KS time p=%.3f, wavelength p=%.3f"
+          % (len(t) / photons, nr / Nr, p_rate, chi2, dof, p_dom, p_t, p_w))
+    assert p_rate > 0.01 and p_dom > 0.01 and p_t > 0.01 and p_w > 0.01
+
+
+def test_cli_flasher_matches_reference_cli():
+    from ppc_amd import _native
+
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "1.6e10", "0"], env=_env(), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    photons = sum(int(m) for m in re.findall(r"photons: (\d+)", r.stderr))
+    rows = [l.split() for l in r.stdout.splitlines() if l.startswith("HIT ")]
+    arr = np.array([[float(v) for v in row[1:5]] for row in rows])
+    assert photons > 1e8
+    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3])
+
+
+_LIB = r'''
+import sys, json
+import numpy as np
+from ppc_amd import api, configs
+D = api.Detector(sys.argv[1])
+D.start(0)
+D.lib.xppc_c_flset(1, 30)
+D.lib.xppc_c_sett(0.0, 0.0, 1.0, 1, 0, 0, 0.0)
+N = 16000000000
+D.lib.xppc_c_flone(N)
+D.lib.xppc_c_eout()
+h = D.hitz()
+photons = D.flasher_photons(N)
+np.savez(sys.argv[2], photons=photons, **h)
+D.stop()
+'''
+
+
+def test_library_flasher_hitz(tmp_path):
+    from ppc_amd import configs
+
+    out = str(tmp_path / "hitz.npz")
+    e = _env()
+    e["PYTHONPATH"] = ROOT
+    r = subprocess.run([sys.executable, "-c", _LIB, configs.ensure("c1"), out], env=e, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    h = np.load(out)
+    assert (h["pmt"] == 0).all()
+    _compare(int(h["photons"]), h["str"], h["dom"], h["t"], h["wv"])

@@ -55,7 +55,7 @@ def test_replay_reference_vectors(case):
     for p in range(procs):
         st[p * n:(p + 1) * n] = G.replay_states(p, n)
     dev.set_states(st[:, 0].astype(np.uint64) | (st[:, 1].astype(np.uint64) << np.uint64(32)))
-    b = U.to_api_bunch(G._bunch(tab, src, total))
+    b = U.to_api_bunch(G._bunch(tab, src, total, det=det))
     s = dev.run([b])
     assert s.photons == total
     hits, hph = dev.hits(with_photon=True)
@@ -101,7 +101,7 @@ def _two_sample_chi2(a, b, min_exp=10):
     return chi2, dof, float(stats.chi2.sf(chi2, dof))
 
 
-SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000}
+SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000, "c3": 10}  # photons; c3: event repeats
 
 
 @pytest.mark.parametrize("case", list(G.CASES))
@@ -112,9 +112,17 @@ def test_statistics_match_reference(case):
     gold = np.load(path)
     det, dev, tab, src = _setup(case, max_hits=1 << 24)
     dev.seed(12345)
-    N = SOURCES[case]
-    b = U.to_api_bunch(G._bunch(tab, src, N))
-    s = dev.run([b])
+    if src[0] == "muon":  # the whole muon event: 1200 segments and its cascades, each bunch repeated
+        bunches = G.muon_event_bunches(
+            lambda x, y, z, t, E, dr, ty, n: det.particle_bunch(x, y, z, t, E, dr, ty, n), 1, 0)
+        for b in bunches:
+            b.num *= SOURCES[case]
+        N = sum(b.num for b in bunches)
+    else:
+        N = SOURCES[case]
+        bunches = [U.to_api_bunch(G._bunch(tab, src, N))]
+    s = dev.run(bunches)
+    assert s.photons == N
     hits = dev.hits()
     dev.close()
     assert s.stored == s.hits

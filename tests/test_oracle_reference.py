@@ -39,7 +39,7 @@ T = U.product_tables(D)
 gold = np.load(sys.argv[2] + "/golden/replay_%s.npz" % case)
 n, procs = int(gold["n"]), int(gold["procs"])
 rm = D.array("rm", np.uint32)
-b = G._bunch(T, src, 1)
+b = G._bunch(T, src, 1, det=D)
 orc = U.oracle()
 cap = 1000
 H = (U.Hit * cap)()
