@@ -155,6 +155,7 @@ typedef struct {
   uint32_t rec_stride;     /* record the MWC state at emission of every rec_stride-th photon (0: off) */
   uint32_t rec_max;        /* capacity of the record buffer */
   int32_t count_events;    /* 1: also count layer crossings, string and DOM tests, scatterings (slower) */
+  int32_t record_starts;   /* 1: keep, per stored hit, the MWC state its photon started from (adjoint) */
 } ppc_hip_config;
 
 /* per-launch results (what gpu/ppc.cu:255-283 reads back) */
@@ -179,6 +180,21 @@ typedef struct {
   uint32_t photon;
   uint32_t x, c, a;
 } ppc_hip_record;
+
+/* start of the photon of a stored hit (record_starts): MWC state and lane */
+typedef struct {
+  uint32_t x, c, lane, pad;
+} ppc_hip_start;
+
+/* one photon to replay in an adjoint pass: its index in the forward launch, the
+ * MWC state and lane it started from, and the sum over its hits of the LLH
+ * weight of the hit DOM */
+typedef struct {
+  uint32_t photon;
+  uint32_t x0, c0, lane;
+  float weight;
+  uint32_t pad[3];
+} ppc_hip_adj_record;
 
 typedef struct ppc_hip_ctx ppc_hip_ctx;
 
@@ -215,6 +231,36 @@ int ppc_hip_wait(ppc_hip_ctx* ctx, ppc_hip_stats* stats);
 int ppc_hip_hits(ppc_hip_ctx* ctx, ppc_hip_hit* hits, uint32_t* photon, uint32_t cap, uint32_t* n);
 /* the replay records of the last launch */
 int ppc_hip_records(ppc_hip_ctx* ctx, ppc_hip_record* rec, uint32_t cap, uint32_t* n);
+
+/* ---- batches over ice variants ---- */
+/* replace the per-layer factors by nvar ice variants, lay[v*size + layer]
+ * (variant 0 first); variants share everything but the layer factors */
+int ppc_hip_set_layers(ppc_hip_ctx* ctx, const ppc_hip_layer* lay, int32_t nvar);
+/* like ppc_hip_launch, with bunch b propagated in ice variant variant[b] */
+int ppc_hip_launch_batch(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, const uint16_t* variant, uint32_t nb,
+                         uint32_t num);
+/* capture the launch of these bunches (counter reset and kernel; the bunches are
+ * copied to buffers of the graph once) into a HIP graph; ppc_hip_graph_launch
+ * replays it and ppc_hip_wait / ppc_hip_hits read its results as after
+ * ppc_hip_launch.  The MWC streams continue, so every replay simulates the same
+ * sources with new random numbers.  ppc_hip_set_layers drops the graph. */
+int ppc_hip_graph_capture(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, const uint16_t* variant, uint32_t nb,
+                          uint32_t num);
+int ppc_hip_graph_launch(ppc_hip_ctx* ctx);
+
+/* ---- gradient of a Poisson log-likelihood of the hit counts ---- */
+/* the start states of the stored hits of the last launch (record_starts) */
+int ppc_hip_hit_starts(ppc_hip_ctx* ctx, ppc_hip_start* starts, uint32_t cap, uint32_t* n);
+/* Replay the recorded photons of a forward launch of these bunches (same num,
+ * ice variant 0) and accumulate into grad[0..size) d(LLH)/d(absorption
+ * parameter a_i) and into grad[size..2*size) d(LLH)/d(scattering parameter
+ * b_i).  weight[s*gsize + dom] is d(LLH)/d(mean hits of dom) for track segment
+ * s < nseg; dpda[i] = d(p_i)/d(a_i) and ds1db[i] = d(s1_i)/d(b_i) relate the
+ * parameters to the factorised layer table.  stats->nonfinite counts replayed
+ * photons whose hits did not reproduce the recorded ones.  Synchronous. */
+int ppc_hip_adjoint(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                    const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                    const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* stats);
 
 /* message of the last failing call (thread local) */
 const char* ppc_hip_last_error(void);

@@ -35,7 +35,9 @@ const ppc_hip_tables* xppc_c_tables(void);
  *   abs_, sca     dense [512][172] ice tables as gpu/ini.cxx:1684-1691 builds them
  *   str, dom      string and DOM number of every in-ice DOM (int32)
  *   wvs           [512][3] wavelength bin centre, low and high edge [nm]
- *   eff, ovr, l0  light-yield scale, oversize factor xR^2, zero string of the 1-d tilt */
+ *   eff, ovr, l0  light-yield scale, oversize factor xR^2, zero string of the 1-d tilt
+ *   par_a, par_b  per layer (layer 0 deepest) the icemodel.dat parameters adust(400), be(400)
+ *   dpda, ds1db   per layer d(lay.p)/d(par_a) and d(lay.s1)/d(par_b) (float) */
 const void* xppc_c_field(const char* name, uint64_t* bytes);
 
 /* lengthen the multiplier table to at least `lanes` entries, continuing the

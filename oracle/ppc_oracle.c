@@ -435,10 +435,33 @@ void orc_emit(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, or
 #define ABS(t, w, J) ((t)->abs_[(w) * ORC_MAXLYS + (J)])
 #define SCA(t, w, J) ((t)->sca[(w) * ORC_MAXLYS + (J)])
 
+/* Path length of the straight segment from layer coordinate z0 (in units of
+ * the layer thickness) to z0 + dz that lies in layer k: the overlap of the
+ * segment with [k-1/2, k+1/2], the outermost layers reaching to infinity, scaled
+ * from layer coordinates to path length (len/|dz|).  Horizontal segments lie in
+ * layer round(z0).  Independent of the layer walk of the transport. */
+static double layer_overlap(int k, int size, double z0, double dz, double len) {
+  if (dz == 0) return k == mini_(maxi_(rn((float)z0), 0), size - 1) ? len : 0;
+  double lo = z0 < z0 + dz ? z0 : z0 + dz, hi = z0 < z0 + dz ? z0 + dz : z0;
+  double a = k == 0 ? -1e30 : k - 0.5, b = k == size - 1 ? 1e30 : k + 0.5;
+  double o = (hi < b ? hi : b) - (lo > a ? lo : a);
+  return o > 0 ? o * len / (hi - lo) : 0;
+}
+
 /* Propagate one emitted photon until it is absorbed, leaves through a DOM
- * (xR=1) or is killed, recording raw hits (pro.cu:482-889). */
+ * (xR=1) or is killed, recording raw hits (pro.cu:482-889).  With A, also
+ * accumulate the score-function gradient of the LLH weight of the photon's hits
+ * (orc_adjoint_photon). */
+static int transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* hits, int cap,
+                     orc_counters* cnt, orc_adjoint* A);
+
 int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* hits, int cap,
                   orc_counters* cnt) {
+  return transport(t, f, s, hits, cap, cnt, NULL);
+}
+
+static int transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* hits, int cap,
+                     orc_counters* cnt, orc_adjoint* A) {
   float r[4], n[4];
   memcpy(r, f->r, sizeof r);
   memcpy(n, f->n, sizeof n);
@@ -655,6 +678,17 @@ int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* h
     } else
       SCA_ = 0, TOT = tot * nr;
 
+    if (A && A->G != 0) { /* path-length scores of the segment, layer by layer */
+      double dz = (double)n[2] * del / edh;
+      for (int k = 0; k < t->size; k++) {
+        double l = layer_overlap(k, t->size, zz, dz, del);
+        if (l > 0) {
+          A->ga[k] -= A->G * nr * A->lk[w] * A->dpda[k] * l;
+          A->gb[k] -= A->G * A->la[w] * A->ds1db[k] * l;
+        }
+      }
+    }
+
     r[0] += del * n[0];
     r[1] += del * n[1];
     r[2] += del * n[2];
@@ -680,6 +714,7 @@ int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* h
         h.dph = atan2f(dy, dx);
         if (nhit < cap) hits[nhit] = h;
         nhit++;
+        if (A && niw < A->nseg) A->G -= A->weight[(size_t)niw * A->gsize + om];
         if (cnt) cnt->hits++;
         if (t->xR == 1)
           TOT = 0;
@@ -768,6 +803,7 @@ int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* h
         sf = t->SF, g = t->G, gr = t->GR;
       else
         sf = t->sf, g = t->g, gr = t->gr;
+      if (A && A->G != 0 && !hole) A->gb[J] += A->G * A->la[w] * A->ds1db[J] / SCA(t, w, J);
       float xi = orc_scatter(sf, g, gr, s);
       float k1 = t->az_k1[J], k2 = t->az_k2[J];
       float okz = k1 * k2;
@@ -798,6 +834,24 @@ int orc_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, o
   orc_pbuf f;
   orc_emit(t, pz, nb, idx, s, &f);
   return orc_transport(t, &f, s, hits, cap, cnt);
+}
+
+int orc_adjoint_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, const orc_rng* s0,
+                       orc_adjoint* A) {
+  orc_hit h[64];
+  orc_rng s = *s0;
+  orc_pbuf f;
+  orc_emit(t, pz, nb, idx, &s, &f);
+  int nh = transport(t, &f, &s, h, 64, NULL, NULL);
+  double G = 0;
+  for (int k = 0; k < nh && k < 64; k++)
+    if (h[k].n < A->nseg) G += A->weight[(size_t)h[k].n * A->gsize + h[k].i];
+  A->G = G;
+  if (G == 0) return nh;
+  s = *s0;
+  orc_emit(t, pz, nb, idx, &s, &f);
+  transport(t, &f, &s, h, 64, NULL, A);
+  return nh;
 }
 
 uint32_t orc_run_xcpu(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t num, uint32_t nthr,

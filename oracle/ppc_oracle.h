@@ -14,6 +14,7 @@
  *   gpu/pro.cu:334-477  photon emission               orc_emit
  *   gpu/pro.cu:482-889  transport step loop           orc_transport
  *   gpu/ppc.cu:432-442  single-thread launch order    orc_run_xcpu
+ *   (no counterpart)    score-function gradient       orc_adjoint_photon
  *
  * Pinning: every function is checked against the reference's own CPU build
  * (oracle/_ref, compiled from gpu/ppc.cu where it lies) photon by photon, on
@@ -133,6 +134,21 @@ typedef struct {
   uint64_t photons, steps, crossings, cells, strings, doms, scatters, hits, draws, killed;
 } orc_counters;
 
+/* gradient of the LLH weight of a photon's hits by per-layer absorption and
+ * scattering parameters (score function of the path density; not part of the
+ * reference).  The caller fills the inputs; ga, gb [size] accumulate. */
+typedef struct {
+  const float* lk;     /* [WNUM] d(abs)/d(p) of a wavelength bin (its lk factor) */
+  const float* la;     /* [WNUM] d(sca)/d(s1) of a wavelength bin (its la factor) */
+  const float* dpda;   /* [size] d(p)/d(a) */
+  const float* ds1db;  /* [size] d(s1)/d(b) */
+  const float* weight; /* [nseg][gsize] LLH weight of a hit of a DOM by segment */
+  uint32_t nseg, gsize;
+  double* ga;          /* [size] += d/d(a) */
+  double* gb;          /* [size] += d/d(b) */
+  double G;            /* working: weight of the hits still to come */
+} orc_adjoint;
+
 /* table handling (the struct is ~1.5 MB: allocate it on the heap) */
 orc_tables* orc_tables_new(void);
 void orc_tables_free(orc_tables* t);
@@ -153,6 +169,10 @@ int orc_transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit* h
 /* one emitted+transported photon from a given state (the replay unit) */
 int orc_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, orc_rng* s,
                orc_hit* hits, int cap, orc_counters* cnt);
+/* the photon from state s0 run twice: once for its hits (their summed weight),
+ * once accumulating the gradient; returns the number of hits */
+int orc_adjoint_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, const orc_rng* s0,
+                       orc_adjoint* A);
 /* the reference's single-thread launch: nthr threads share one MWC stream and
  * thread k emits then transports photons k, k+nthr, ... (gpu/ppc.cu:439-440) */
 uint32_t orc_run_xcpu(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t num, uint32_t nthr,

@@ -1,0 +1,126 @@
+"""Gradient of a likelihood of DOM hit counts by the per-layer ice parameters.
+
+The forward simulation propagates photons in ice whose layer i has absorption
+parameter a_i (icemodel.dat adust(400)) and scattering parameter b_i
+(be(400)); the layer factors of the kernel depend on them affinely
+(p_i = Dc*a_i + ..., s1_i = b_i/(1-g) + ..., gpu/ini.cxx:1630-1640).  The
+expected hit count of DOM d in track segment s is mu[s, d] = scale * n[s, d],
+n the simulated count.
+
+The gradient is the score-function (likelihood-ratio) estimator of the path
+density: a hit's weight times d log p(path up to the hit)/d theta, where along
+every straight segment of length l_k in layer k the absorption score is
+-nr*lk*dp/da*l_k and the scattering score -la*ds1/db*l_k, plus
+la*ds1/db / sca_J at every scattering in layer J.  The backward pass replays
+only the photons that hit, each from the MWC state it started from (kept by the
+forward launch), with the weight of the hits it still has to make; it runs on
+the GPU (ppc_hip_adjoint) and accumulates in FP64.
+
+    sim = HitSimulation(det, bunches, scale=1.0)
+    a = torch.tensor(par_a, requires_grad=True); b = ...
+    mu = ExpectedHits.apply(a, b, sim)          # [nseg, gsize]
+    llh = poisson_llh(mu, data)
+    llh.backward()                              # a.grad, b.grad
+"""
+import numpy as np
+import torch
+
+from . import api
+
+
+def adjoint_records(hits, photon, starts, weight):
+    """One adjoint record per photon with a weighted hit: its start state and the
+    summed weight[seg, dom] of its stored hits (hits of segments beyond
+    weight's first axis weigh 0)."""
+    nseg = weight.shape[0]
+    seg = hits["n"].astype(np.int64)
+    ok = seg < nseg
+    w = np.zeros(len(hits), dtype=np.float64)
+    w[ok] = weight[seg[ok], hits["i"][ok]]
+    rec = np.zeros(0, dtype=api.ADJ_RECORD_DTYPE)
+    if len(hits) == 0:
+        return rec
+    order = np.argsort(photon, kind="stable")
+    ph = photon[order]
+    first = np.flatnonzero(np.r_[True, ph[1:] != ph[:-1]])
+    wsum = np.add.reduceat(w[order], first)
+    wabs = np.add.reduceat(np.abs(w[order]), first)
+    keep = wabs > 0
+    sel = order[first][keep]
+    rec = np.zeros(int(keep.sum()), dtype=api.ADJ_RECORD_DTYPE)
+    rec["photon"] = photon[sel]
+    rec["x0"] = starts["x"][sel]
+    rec["c0"] = starts["c"][sel]
+    rec["lane"] = starts["lane"][sel]
+    rec["weight"] = wsum[keep]
+    return rec
+
+
+class HitSimulation:
+    """A device context, the bunches of one event and the bookkeeping between a
+    forward launch and its adjoint.  The ice parameters enter through
+    Detector.layers_for(a, b); everything else of the ice stays as loaded."""
+
+    def __init__(self, det, bunches, scale=1.0, nseg=None, device=0, **device_kw):
+        self.det = det
+        self.bunches = [bunches] if isinstance(bunches, api.Bunch) else list(bunches)
+        self.num = sum(b.num for b in self.bunches)
+        self.scale = float(scale)
+        self.gsize = int(det.scalar("gsize", np.int32))
+        self.nseg = nseg if nseg is not None else 1 + max(b.q for b in self.bunches)
+        q = det.ice_parameters()
+        self.dpda, self.ds1db = q["dpda"], q["ds1db"]
+        self.par_a, self.par_b = q["par_a"], q["par_b"]
+        self.dev = api.Device(device, record_starts=True, **device_kw)
+        self.dev.upload(det)
+
+    def close(self):
+        self.dev.close()
+
+    def forward(self, a, b):
+        """simulate with per-layer a, b; returns (mu [nseg, gsize], record of the launch)"""
+        self.dev.set_layers(self.det.layers_for(a, b))
+        self.dev.launch(self.bunches, self.num)
+        st = self.dev.wait()
+        if st.hits > st.stored:
+            raise api.PPCError("%d hits exceed the hit buffer (%d); open the device with a larger max_hits"
+                               % (st.hits, st.stored))
+        hits, photon = self.dev.hits(with_photon=True)
+        starts = self.dev.hit_starts()
+        seg = hits["n"].astype(np.int64)
+        ok = seg < self.nseg
+        n = np.zeros((self.nseg, self.gsize), dtype=np.float64)
+        np.add.at(n, (seg[ok], hits["i"][ok].astype(np.int64)), 1.0)
+        return n * self.scale, {"hits": hits, "photon": photon, "starts": starts, "stats": st}
+
+    def backward(self, launch, grad_mu):
+        """d(L)/d(a), d(L)/d(b) given dL/dmu [nseg, gsize] of the launch's mu"""
+        weight = np.asarray(grad_mu, dtype=np.float64) * self.scale
+        rec = adjoint_records(launch["hits"], launch["photon"], launch["starts"], weight)
+        ga, gb, st = self.dev.adjoint(self.bunches, self.num, rec, weight.astype(np.float32), self.dpda, self.ds1db)
+        self.last_adjoint = {"records": len(rec), "missed": st.nonfinite, "ms": st.device_ms}
+        return ga, gb
+
+
+class ExpectedHits(torch.autograd.Function):
+    """mu[s, d] = scale * (simulated hits of DOM d in segment s) as a function of the
+    per-layer ice parameters a, b (float tensors [size]); the backward pass is the
+    GPU adjoint of that very launch (same photons, same random numbers)."""
+
+    @staticmethod
+    def forward(ctx, a, b, sim):
+        mu, launch = sim.forward(a.detach().cpu().double().numpy(), b.detach().cpu().double().numpy())
+        ctx.sim, ctx.launch = sim, launch
+        ctx.dtypes = (a.dtype, b.dtype)
+        return torch.from_numpy(mu)
+
+    @staticmethod
+    def backward(ctx, grad_mu):
+        ga, gb = ctx.sim.backward(ctx.launch, grad_mu.detach().cpu().double().numpy())
+        return torch.from_numpy(ga).to(ctx.dtypes[0]), torch.from_numpy(gb).to(ctx.dtypes[1]), None
+
+
+def poisson_llh(mu, k, floor=1e-9):
+    """sum over bins of k log mu - mu (the hit-count log-likelihood up to a constant)"""
+    k = torch.as_tensor(k, dtype=mu.dtype)
+    return (k * torch.log(mu.clamp_min(floor)) - mu).sum()

@@ -44,12 +44,17 @@ class Hit(ctypes.Structure):
 HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"), ("pph", "<f4"),
                       ("dth", "<f4"), ("dph", "<f4")])
 RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<u4")])
+START_DTYPE = np.dtype([("x", "<u4"), ("c", "<u4"), ("lane", "<u4"), ("pad", "<u4")])
+ADJ_RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x0", "<u4"), ("c0", "<u4"), ("lane", "<u4"), ("weight", "<f4"),
+                             ("pad", "<u4", (3,))])
+LAYER_DTYPE = np.dtype([("s0", "<f4"), ("s1", "<f4"), ("p", "<f4"), ("t", "<f4")])
 
 
 class Config(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32), ("refill", ctypes.c_int32),
                 ("deterministic", ctypes.c_int32), ("max_bunches", ctypes.c_uint32), ("max_hits", ctypes.c_uint32),
-                ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32), ("count_events", ctypes.c_int32)]
+                ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32), ("count_events", ctypes.c_int32),
+                ("record_starts", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -91,6 +96,13 @@ def hip():
         lib.ppc_hip_wait.argtypes = [c_p, ctypes.POINTER(Stats)]
         lib.ppc_hip_hits.argtypes = [c_p, c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         lib.ppc_hip_records.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_set_layers.argtypes = [c_p, c_p, ctypes.c_int32]
+        lib.ppc_hip_launch_batch.argtypes = [c_p, ctypes.POINTER(Bunch), c_p, ctypes.c_uint32, ctypes.c_uint32]
+        lib.ppc_hip_graph_capture.argtypes = [c_p, ctypes.POINTER(Bunch), c_p, ctypes.c_uint32, ctypes.c_uint32]
+        lib.ppc_hip_graph_launch.argtypes = [c_p]
+        lib.ppc_hip_hit_starts.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_adjoint.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32, c_p,
+                                        ctypes.c_uint32, c_p, ctypes.c_uint32, c_p, c_p, c_p, ctypes.POINTER(Stats)]
         _hip = lib
     return _hip
 
@@ -200,6 +212,25 @@ class Detector:
         self.lib.xppc_c_hitz(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("str", "dom", "pmt", "t", "wv")], n)
         return out
 
+    def layers(self):
+        """per-layer factors (s0, s1, p, t) of the ice tables, layer 0 deepest"""
+        return self.array("lay", LAYER_DTYPE)
+
+    def ice_parameters(self):
+        """per layer: icemodel.dat adust(400) and be(400), and d(p)/d(adust), d(s1)/d(be)"""
+        return {k: self.array(k, np.float32) for k in ("par_a", "par_b", "dpda", "ds1db")}
+
+    def layers_for(self, a, b):
+        """the layer factors of the same ice with per-layer adust(400) = a and be(400) = b
+        (p and s1 are affine in them: gpu/ini.cxx:1630-1640)"""
+        lay = self.layers()
+        q = self.ice_parameters()
+        a = np.asarray(a, dtype=np.float64)
+        b = np.asarray(b, dtype=np.float64)
+        lay["p"] = lay["p"] + q["dpda"] * (a - q["par_a"])
+        lay["s1"] = lay["s1"] + q["ds1db"] * (b - q["par_b"])
+        return lay
+
     def zshift(self, x, y, z):
         dz = ctypes.c_float()
         s = self.lib.xppc_c_zshift(x, y, z, ctypes.byref(dz))
@@ -216,10 +247,10 @@ class Device:
     """One GPU context of libppc_hip.so."""
 
     def __init__(self, device=0, blocks=0, threads=0, refill=0, deterministic=False, max_bunches=0, max_hits=0,
-                 rec_stride=0, rec_max=0, count_events=False):
+                 rec_stride=0, rec_max=0, count_events=False, record_starts=False):
         self.lib = hip()
         self.cfg = Config(blocks, threads, refill, 1 if deterministic else 0, max_bunches, max_hits, rec_stride,
-                          rec_max, 1 if count_events else 0)
+                          rec_max, 1 if count_events else 0, 1 if record_starts else 0)
         self.ctx = ctypes.c_void_p()
         _check(self.lib.ppc_hip_open(device, ctypes.byref(self.cfg), ctypes.byref(self.ctx)))
         n = ctypes.c_int32()
@@ -293,3 +324,68 @@ class Device:
     def run(self, bunches, num=None):
         self.launch(bunches, num)
         return self.wait()
+
+    # ---- ice variants, graphs ----
+    def set_layers(self, lay):
+        """lay: LAYER_DTYPE array [nvar][size] (or [size]) of per-layer factors"""
+        lay = np.ascontiguousarray(lay, dtype=LAYER_DTYPE)
+        nvar = 1 if lay.ndim == 1 else lay.shape[0]
+        _check(self.lib.ppc_hip_set_layers(self.ctx, lay.ctypes.data_as(ctypes.c_void_p), nvar))
+
+    @staticmethod
+    def _bunches(bunches, variant):
+        if isinstance(bunches, Bunch):
+            bunches = [bunches]
+        arr = (Bunch * len(bunches))(*bunches)
+        v = None
+        if variant is not None:
+            v = np.ascontiguousarray(variant, dtype=np.uint16)
+            if len(v) != len(bunches):
+                raise ValueError("one ice variant per bunch")
+        return arr, v
+
+    def launch_batch(self, bunches, variant, num=None):
+        arr, v = self._bunches(bunches, variant)
+        if num is None:
+            num = sum(b.num for b in arr)
+        _check(self.lib.ppc_hip_launch_batch(self.ctx, arr, v.ctypes.data_as(ctypes.c_void_p) if v is not None
+                                             else None, len(arr), num))
+
+    def graph_capture(self, bunches, variant=None, num=None):
+        arr, v = self._bunches(bunches, variant)
+        if num is None:
+            num = sum(b.num for b in arr)
+        self._graph_keep = (arr, v)
+        _check(self.lib.ppc_hip_graph_capture(self.ctx, arr, v.ctypes.data_as(ctypes.c_void_p) if v is not None
+                                              else None, len(arr), num))
+
+    def graph_launch(self):
+        _check(self.lib.ppc_hip_graph_launch(self.ctx))
+
+    # ---- adjoint ----
+    def hit_starts(self):
+        st = Stats()
+        _check(self.lib.ppc_hip_wait(self.ctx, ctypes.byref(st)))
+        m = int(st.stored)
+        a = np.zeros(m, dtype=START_DTYPE)
+        n = ctypes.c_uint32()
+        _check(self.lib.ppc_hip_hit_starts(self.ctx, a.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
+        return a[:n.value]
+
+    def adjoint(self, bunches, num, records, weight, dpda, ds1db):
+        """replay records (ADJ_RECORD_DTYPE) of a forward launch of these bunches and
+        return (d/d adust[size], d/d be[size], stats); weight [nseg][gsize] float32"""
+        arr, _ = self._bunches(bunches, None)
+        records = np.ascontiguousarray(records, dtype=ADJ_RECORD_DTYPE)
+        weight = np.ascontiguousarray(weight, dtype=np.float32)
+        nseg = weight.shape[0] if weight.ndim == 2 else 1
+        dpda = np.ascontiguousarray(dpda, dtype=np.float32)
+        ds1db = np.ascontiguousarray(ds1db, dtype=np.float32)
+        size = len(dpda)
+        grad = np.zeros(2 * size, dtype=np.float64)
+        st = Stats()
+        _check(self.lib.ppc_hip_adjoint(self.ctx, arr, len(arr), num, records.ctypes.data_as(ctypes.c_void_p),
+                                        len(records), weight.ctypes.data_as(ctypes.c_void_p), nseg,
+                                        dpda.ctypes.data_as(ctypes.c_void_p), ds1db.ctypes.data_as(ctypes.c_void_p),
+                                        grad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
+        return grad[:size], grad[size:], st

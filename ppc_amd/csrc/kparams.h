@@ -78,6 +78,15 @@ struct KParams {
   uint32_t rec_stride, rec_max;
   int refill, deterministic;
   uint32_t chunk;  // photons claimed per atomic on the global counter
+  const uint16_t* bvar;  // ice variant of each bunch (nullptr: variant 0)
+  uint4* hit_start;      // per hit: MWC state before emission (x, c) and lane, or nullptr
+  // adjoint replay
+  const ppc_hip_adj_record* adj_rec;
+  uint32_t adj_nrec, adj_nseg;
+  const float* adj_weight;  // [adj_nseg][gsize] d(LLH)/d(hits of a DOM)
+  const float* adj_dpda;    // [size] d(p)/d(absorption parameter)
+  const float* adj_ds1db;   // [size] d(s1)/d(scattering parameter)
+  double* adj_grad;         // [2][size]
   uint32_t lanes;
 };
 

@@ -18,7 +18,8 @@
 
 namespace ppc {
 hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int threads, size_t lds_bytes,
-                            hipStream_t stream, bool count);
+                            hipStream_t stream, bool count, bool adjoint);
+hipError_t launch_reset(Counters* ctr, hipStream_t stream);
 }
 
 using ppc::KParams;
@@ -87,7 +88,58 @@ struct ppc_hip_ctx {
   size_t lds_bytes = 0;
   KParams P{};
   uint32_t last_num = 0;
+  std::vector<uint8_t> img_base;  // LDS image without the layer table
+  int nvar = 1;                   // ice variants in the layer table
+  uint16_t* d_bvar = nullptr;     // per-bunch ice variant of a batch launch
+  uint16_t* h_bvar = nullptr;
+  uint4* d_hit_start = nullptr;   // per-hit start state (record_starts)
+  // adjoint
+  ppc_hip_adj_record* d_adj_rec = nullptr;
+  uint32_t adj_cap = 0;
+  float* d_adj_w = nullptr;
+  size_t adj_wcap = 0;
+  float* d_adj_dp = nullptr;
+  double* d_adj_grad = nullptr;
+  hipGraph_t graph = nullptr;  // ppc_hip_graph_capture: the graph and the buffers it reads
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_num = 0;
+  ppc_hip_bunch* g_pz = nullptr;
+  uint32_t* g_pstart = nullptr;
+  uint16_t* g_bvar = nullptr;
+  KParams* g_P = nullptr;
 };
+
+namespace {
+void drop_graph(ppc_hip_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (void* p : {(void*)c->g_pz, (void*)c->g_pstart, (void*)c->g_bvar, (void*)c->g_P})
+    if (p) (void)hipFree(p);
+  c->graph_exec = nullptr, c->graph = nullptr;
+  c->g_pz = nullptr, c->g_pstart = nullptr, c->g_bvar = nullptr, c->g_P = nullptr;
+}
+
+// append the layer table of nvar ice variants ([nvar][size]) to the LDS image and upload it
+int set_layers(ppc_hip_ctx* c, const ppc_hip_layer* lay, int nvar) {
+  KParams& P = c->P;
+  std::vector<uint8_t> img = c->img_base;
+  P.off_lay = (int)img.size();
+  img.resize(img.size() + sizeof(ppc_hip_layer) * P.size * nvar);
+  memcpy(img.data() + P.off_lay, lay, sizeof(ppc_hip_layer) * P.size * nvar);
+  img.resize((img.size() + 15) & ~size_t(15));
+  if (img.size() + sizeof(float) * 2 * P.size > 64 * 1024)
+    return fail(PPC_HIP_E_CAPACITY, "LDS image of %zu bytes too large (%d ice variants)", img.size(), nvar);
+  HIPCHK(hipStreamSynchronize(c->stream));  // a launch in flight may still read the old image
+  if (c->d_image) (void)hipFree(c->d_image), c->d_image = nullptr;
+  HIPCHK(dalloc(&c->d_image, img.size() / 16));
+  HIPCHK(hipMemcpy(c->d_image, img.data(), img.size(), hipMemcpyHostToDevice));
+  c->lds_bytes = img.size();
+  P.image_words = (int)(img.size() / 16);
+  P.image = c->d_image;
+  c->nvar = nvar;
+  return PPC_HIP_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -161,6 +213,9 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(hipHostMalloc((void**)&c->h_ctr, sizeof(ppc::Counters), hipHostMallocDefault));
   chk(hipHostMalloc((void**)&c->h_P, sizeof(KParams), hipHostMallocDefault));
   chk(dalloc(&c->d_P, 1));
+  chk(dalloc(&c->d_bvar, k.max_bunches));
+  chk(hipHostMalloc((void**)&c->h_bvar, sizeof(uint16_t) * k.max_bunches, hipHostMallocDefault));
+  if (k.record_starts) chk(dalloc(&c->d_hit_start, k.max_hits));
   if (e != hipSuccess) {
     int code = e == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE;
     ppc_hip_close(c);
@@ -176,9 +231,11 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P})
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hit_start,
+                  (void*)c->d_adj_rec, (void*)c->d_adj_w, (void*)c->d_adj_dp, (void*)c->d_adj_grad})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr, (void*)c->h_P})
+  drop_graph(c);
+  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr, (void*)c->h_P, (void*)c->h_bvar})
     if (p) (void)hipHostFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -274,7 +331,6 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     if (bytes) memcpy(img.data() + off, src, bytes);
     return (int)off;
   };
-  P.off_lay = put(t->lay, sizeof(ppc_hip_layer) * t->size);
   P.off_az = put(t->az, sizeof(ppc_hip_aniz) * t->size);
   P.off_lr = put(t->lr, sizeof(float) * PPC_HIP_LMAX);
   P.off_mcol = put(t->mcol, sizeof(int32_t) * PPC_HIP_CTX * PPC_HIP_CTY);
@@ -303,15 +359,15 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.lp_in_lds = lp.size() * sizeof(float2) <= 16384 ? 1 : 0;
   if (P.lp_in_lds) P.off_lp = put(lp.data(), sizeof(float2) * lp.size());
   img.resize((img.size() + 15) & ~size_t(15));
-  c->lds_bytes = img.size();
-  P.image_words = (int)(img.size() / 16);
-  if (c->lds_bytes > 64 * 1024) return fail(PPC_HIP_E_CAPACITY, "LDS image of %zu bytes too large", c->lds_bytes);
+  c->img_base = img;  // the layer table (ice variants) is appended by set_layers
 
   for (void* p : {(void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv, (void*)c->d_dom, (void*)c->d_domF})
     if (p) (void)hipFree(p);
   c->d_image = nullptr, c->d_lp = nullptr, c->d_wv = nullptr, c->d_dom = nullptr, c->d_domF = nullptr;
-  HIPCHK(dalloc(&c->d_image, P.image_words));
-  HIPCHK(hipMemcpy(c->d_image, img.data(), img.size(), hipMemcpyHostToDevice));
+  {
+    int rc = set_layers(c, t->lay, 1);
+    if (rc != PPC_HIP_OK) return rc;
+  }
   HIPCHK(dalloc(&c->d_lp, lp.size()));
   HIPCHK(hipMemcpy(c->d_lp, lp.data(), sizeof(float2) * lp.size(), hipMemcpyHostToDevice));
   HIPCHK(dalloc(&c->d_wv, PPC_HIP_WNUM));
@@ -334,7 +390,6 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   }
   HIPCHK(hipMemcpy(c->d_rm, t->rm, sizeof(uint32_t) * c->lanes, hipMemcpyHostToDevice));
 
-  P.image = c->d_image;
   P.lp = c->d_lp;
   P.wv = c->d_wv;
   P.dom = c->d_dom;
@@ -353,6 +408,7 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.refill = c->cfg.refill;
   P.deterministic = c->cfg.deterministic;
   P.lanes = (uint32_t)c->lanes;
+  P.hit_start = c->d_hit_start;
   c->uploaded = true;
   return ppc_hip_seed(c, 0);
 }
@@ -379,8 +435,10 @@ int ppc_hip_get_states(ppc_hip_ctx* c, uint64_t* states, int32_t lanes) {
   return PPC_HIP_OK;
 }
 
-int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num) {
-  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+namespace {
+
+// check the bunches and copy them (and their ice variants) into pinned staging
+int stage(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant, uint32_t nb, uint32_t num) {
   if (!c->uploaded) return fail(PPC_HIP_E_STATE, "launch before upload");
   if (nb > c->cfg.max_bunches) return fail(PPC_HIP_E_CAPACITY, "%u bunches > capacity %u", nb, c->cfg.max_bunches);
   if (num > 0 && (nb == 0 || !bunches)) return fail(PPC_HIP_E_ARG, "photons without bunches");
@@ -400,32 +458,193 @@ int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, ui
     const ppc_hip_bunch& p = bunches[b];
     if (p.type > 0 && p.u.l.ka > 999.f && p.n[3] != 0 && (p.u.l.fla < 0 || p.u.l.fla >= c->P.gsize))
       return fail(PPC_HIP_E_ARG, "bunch %u: flasher DOM %d outside the geometry", b, (int)p.u.l.fla);
+    if (variant && variant[b] >= c->nvar)
+      return fail(PPC_HIP_E_ARG, "bunch %u: ice variant %u of %d", b, (unsigned)variant[b], c->nvar);
   }
   c->h_pstart[nb] = (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
   if (nb) memcpy(c->h_pz, bunches, sizeof(ppc_hip_bunch) * nb);
-  if (nb) {
-    HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
-  }
+  if (nb && variant) memcpy(c->h_bvar, variant, sizeof(uint16_t) * nb);
   *c->h_P = c->P;
   c->h_P->nb = nb, c->h_P->num = num;
+  c->h_P->bvar = variant ? c->d_bvar : nullptr;
   {  // claim chunks of photons per wave: large enough to keep the global counter cold, small
      // enough that the last chunks of a launch still spread over all waves
     const uint32_t waves = (uint32_t)c->lanes / 64;
     uint64_t ch = num / ((uint64_t)waves * 16 + 1);
     c->h_P->chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ch, 64), 1024);
   }
+  return PPC_HIP_OK;
+}
+
+// copies, counter reset and kernel on the context's stream (graph-capturable: no memset nodes)
+int enqueue(ppc_hip_ctx* c, uint32_t nb, bool variants, bool timed) {
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+    if (variants)
+      HIPCHK(hipMemcpyAsync(c->d_bvar, c->h_bvar, sizeof(uint16_t) * nb, hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipEventRecord(c->ev_copy, c->stream));
-  c->copy_pending = true;
-  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(ppc::Counters), c->stream));
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (num > 0)
+  if (timed) HIPCHK(hipEventRecord(c->ev_copy, c->stream));
+  HIPCHK(ppc::launch_reset(c->d_ctr, c->stream));
+  if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (c->h_P->num > 0)
     HIPCHK(ppc::launch_propagate(*c->h_P, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream,
-                                 c->cfg.count_events != 0));
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
+                                 c->cfg.count_events != 0, false));
+  if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+  return PPC_HIP_OK;
+}
+
+}  // namespace
+
+int ppc_hip_launch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num) {
+  return ppc_hip_launch_batch(c, bunches, nullptr, nb, num);
+}
+
+int ppc_hip_launch_batch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant, uint32_t nb,
+                         uint32_t num) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  int rc = stage(c, bunches, variant, nb, num);
+  if (rc != PPC_HIP_OK) return rc;
+  rc = enqueue(c, nb, variant != nullptr, true);
+  if (rc != PPC_HIP_OK) return rc;
+  c->copy_pending = true;
   c->launched = true;
   c->last_num = num;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_set_layers(ppc_hip_ctx* c, const ppc_hip_layer* lay, int32_t nvar) {
+  if (!c || !lay || nvar < 1 || nvar > 0xffff) return fail(PPC_HIP_E_ARG, "bad layer table");
+  if (!c->uploaded) return fail(PPC_HIP_E_STATE, "set_layers before upload");
+  HIPCHK(hipSetDevice(c->device));
+  drop_graph(c);  // a captured graph reads the layer table it was captured with
+  return set_layers(c, lay, nvar);
+}
+
+int ppc_hip_graph_capture(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant, uint32_t nb,
+                          uint32_t num) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  int rc = stage(c, bunches, variant, nb, num);
+  if (rc != PPC_HIP_OK) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->copy_pending = false;
+  drop_graph(c);
+  // the graph owns its bunches and parameter block, so that later launches (which
+  // reuse the staging buffers) leave its replays unchanged; it holds kernels only
+  const uint32_t nbs = std::max<uint32_t>(nb, 1);
+  HIPCHK(dalloc(&c->g_pz, nbs));
+  HIPCHK(dalloc(&c->g_pstart, nbs + 1));
+  HIPCHK(dalloc(&c->g_bvar, nbs));
+  HIPCHK(dalloc(&c->g_P, 1));
+  KParams& H = *c->h_P;
+  H.pz = c->g_pz, H.pstart = c->g_pstart;
+  H.bvar = variant ? c->g_bvar : nullptr;
+  if (nb) {
+    HIPCHK(hipMemcpy(c->g_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice));
+    if (variant) HIPCHK(hipMemcpy(c->g_bvar, c->h_bvar, sizeof(uint16_t) * nb, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c->g_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->g_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice));
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = ppc::launch_reset(c->d_ctr, c->stream);
+  if (e == hipSuccess && num > 0)
+    e = ppc::launch_propagate(H, c->g_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream,
+                              c->cfg.count_events != 0, false);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    HIPCHK(e2);
+  }
+  c->graph = g;
+  HIPCHK(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  c->graph_num = num;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_graph_launch(ppc_hip_ctx* c) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  if (!c->graph_exec) return fail(PPC_HIP_E_STATE, "no captured graph");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->launched = true;
+  c->last_num = c->graph_num;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_hit_starts(ppc_hip_ctx* c, ppc_hip_start* starts, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (!c->d_hit_start) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "hit starts without launch");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  uint32_t m = std::min<uint32_t>(std::min<uint32_t>(c->h_ctr->hidx, c->cfg.max_hits), cap);
+  if (m > 0 && starts) HIPCHK(hipMemcpy(starts, c->d_hit_start, sizeof(ppc_hip_start) * m, hipMemcpyDeviceToHost));
+  *n = m;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                    const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                    const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* st) {
+  if (!c || !grad || !dpda || !ds1db || (nrec && !rec) || (nseg && !weight)) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (c->P.hr > 0) return fail(PPC_HIP_E_ARG, "the adjoint does not model hole ice");
+  for (uint32_t k = 0; k < nrec; k++)
+    if (rec[k].lane >= (uint32_t)c->lanes || rec[k].photon >= num)
+      return fail(PPC_HIP_E_ARG, "adjoint record %u: photon %u lane %u out of range", k, rec[k].photon, rec[k].lane);
+  int rc = stage(c, bunches, nullptr, nb, num);
+  if (rc != PPC_HIP_OK) return rc;
+  const int size = c->P.size;
+  if (nrec > c->adj_cap) {
+    if (c->d_adj_rec) (void)hipFree(c->d_adj_rec);
+    c->d_adj_rec = nullptr;
+    HIPCHK(dalloc(&c->d_adj_rec, nrec));
+    c->adj_cap = nrec;
+  }
+  const size_t wn = (size_t)std::max<uint32_t>(nseg, 1) * c->P.gsize;
+  if (wn > c->adj_wcap) {
+    if (c->d_adj_w) (void)hipFree(c->d_adj_w);
+    c->d_adj_w = nullptr;
+    HIPCHK(dalloc(&c->d_adj_w, wn));
+    c->adj_wcap = wn;
+  }
+  if (!c->d_adj_dp) HIPCHK(dalloc(&c->d_adj_dp, 2 * PPC_HIP_MAXLYS));
+  if (!c->d_adj_grad) HIPCHK(dalloc(&c->d_adj_grad, 2 * PPC_HIP_MAXLYS));
+  if (nrec) HIPCHK(hipMemcpy(c->d_adj_rec, rec, sizeof(ppc_hip_adj_record) * nrec, hipMemcpyHostToDevice));
+  if (nseg) HIPCHK(hipMemcpy(c->d_adj_w, weight, sizeof(float) * nseg * c->P.gsize, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_adj_dp, dpda, sizeof(float) * size, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_adj_dp + size, ds1db, sizeof(float) * size, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_adj_grad, 0, sizeof(double) * 2 * size));
+  KParams& H = *c->h_P;
+  H.adj_rec = c->d_adj_rec, H.adj_nrec = nrec, H.adj_nseg = nseg;
+  H.adj_weight = c->d_adj_w, H.adj_dpda = c->d_adj_dp, H.adj_ds1db = c->d_adj_dp + size;
+  H.adj_grad = c->d_adj_grad;
+  H.hit_start = nullptr;
+  HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(ppc::launch_reset(c->d_ctr, c->stream));
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (nrec)
+    HIPCHK(ppc::launch_propagate(H, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, false, true));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(grad, c->d_adj_grad, sizeof(double) * 2 * size, hipMemcpyDeviceToHost));
+  if (st) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const ppc::Counters& k = *c->h_ctr;
+    *st = ppc_hip_stats{};
+    st->photons = k.photons, st->steps = k.steps, st->killed = k.killed, st->nonfinite = k.nonfinite;
+    st->device_ms = ms, st->lanes = c->lanes;
+  }
+  c->launched = false;  // the hit buffer belongs to the last forward launch, which it no longer matches
   return PPC_HIP_OK;
 }
 
@@ -456,6 +675,7 @@ int ppc_hip_hits(ppc_hip_ctx* c, ppc_hip_hit* hits, uint32_t* photon, uint32_t c
   if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
   if (!c->launched) return fail(PPC_HIP_E_STATE, "hits without launch");
   HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   uint32_t stored = std::min<uint32_t>(c->h_ctr->hidx, c->cfg.max_hits);
   uint32_t m = std::min(stored, cap);

@@ -156,12 +156,13 @@ struct Tabs {
 // wavelength factors of the photon being propagated
 struct Wave {
   float la, lk, abl;
+  int lay;  // first row of the photon's ice variant in the layer table
   __device__ __forceinline__ float sca(const Tabs& T, int J) const {
-    float4 l = T.lay[J];
+    float4 l = T.lay[lay + J];
     return l.x + l.y * la;
   }
   __device__ __forceinline__ float abs(const Tabs& T, int J) const {
-    float4 l = T.lay[J];
+    float4 l = T.lay[lay + J];
     return l.z * lk + l.w * abl;
   }
 };
@@ -245,6 +246,30 @@ __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const f
   }
 }
 
+// Lengths of a straight path of length L from layer coordinate z (layer I)
+// along a direction with |cos zenith| = anz, split over the layers it crosses:
+// the partial first layer, whole layers in between, the rest in the last
+// layer -- the split of gpu/pro.cu:681-696.  f(k, l) is called per layer.
+template <class F>
+__device__ __forceinline__ void split_layers(const KParams& P, float z, int I, int sign, float anz, float edh, float L,
+                                             F f) {
+  const float y = z + sign * anz * L / edh;
+  const int J = min(max(rn(y), 0), P.size - 1);
+  if (I == J) {
+    f(I, L);
+    return;
+  }
+  const float full = edh / anz;
+  float l = (0.5f - sign * (z - I)) * full, used = 0;
+  for (int k = I; k != J; k += sign) {
+    if (l > L - used) l = L - used;
+    f(k, l);
+    used += l;
+    l = full;
+  }
+  f(J, fmaxf(L - used, 0.f));
+}
+
 // ---------------------------------------------------------------- lane state
 
 struct Photon {
@@ -252,6 +277,7 @@ struct Photon {
   float TOT, SCA;
   int old, ofla;
   uint32_t seg, wbin, idx;
+  uint32_t x0, c0;  // MWC state before emission (replay records)
   Wave w;
 };
 
@@ -389,6 +415,7 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
   ph.seg = p.q;
   ph.wbin = wv.wvl;
   ph.w.la = wv.la, ph.w.lk = wv.lk, ph.w.abl = wv.abl;
+  ph.w.lay = P.bvar ? (int)P.bvar[lo] * P.size : 0;
   ph.ofla = ofla;
   return fla;
 }
@@ -398,11 +425,19 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
 // The parameter block is read through a pointer into device memory: uniform
 // loads become scalar loads the compiler can repeat, where a by-value argument
 // of this size would be held in SGPRs and spilled into VGPR lanes.
-template <int TILT, bool EXT, bool COUNT>
+// ADJ: replay the photons of adjoint records (each from its recorded MWC state)
+// and accumulate d(LLH)/d(per-layer absorption, scattering) instead of hits:
+// along every path segment the score of the path density, -nr*d(abs)/da*l for
+// absorption and d(sca)/db*(1/sca at a scattering - l) for scattering, weighted
+// by the LLH weight of the hits the photon still makes after the segment.
+template <int TILT, bool EXT, bool COUNT, bool ADJ>
 __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp) {
   const KParams& P = *Pp;
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
+  float* gacc = reinterpret_cast<float*>(lds + P.image_words);  // ADJ: [2][size] block sums
+  if (ADJ)
+    for (int k = threadIdx.x; k < 2 * P.size; k += blockDim.x) gacc[k] = 0.f;
   __syncthreads();
 
   const char* base = reinterpret_cast<const char*>(lds);
@@ -436,6 +471,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
   const uint32_t chunk = P.chunk;
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;  // event counts (COUNT builds only)
+  float G = 0.f;  // ADJ: LLH weight of the hits the photon has still to make
+  const uint32_t nwork = ADJ ? P.adj_nrec : P.num;
 
   while (true) {
     // -------- refill idle lanes --------
@@ -444,7 +481,19 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       const int nidle = __popcll(idle);
       if (nidle >= P.refill || nidle == __popcll(__ballot(true))) {
         uint32_t i = 0xffffffffu;
-        if (P.deterministic) {
+        if (ADJ) {  // one record after the other per lane
+          if (!live) {
+            const uint32_t k = next_static;
+            next_static += P.lanes;
+            if (k < nwork) {
+              const ppc_hip_adj_record rc = P.adj_rec[k];
+              s.x = rc.x0, s.c = rc.c0, s.a = P.rm[rc.lane];
+              G = rc.weight;
+              i = rc.photon;
+            }
+          }
+          done = !__any(next_static < nwork);
+        } else if (P.deterministic) {
           if (!live) i = next_static, next_static += P.lanes;
           done = !__any(next_static < P.num);
         } else {
@@ -471,7 +520,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
           }
         }
         if (!live && i < P.num) {
-          if (P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
+          ph.x0 = s.x, ph.c0 = s.c;
+          if (!ADJ && P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
             ppc_hip_record rc;
             rc.photon = i, rc.x = s.x, rc.c = s.c, rc.a = s.a;
             P.rec[i / P.rec_stride] = rc;
@@ -697,11 +747,29 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
     } else
       SCA = 0, TOT = tot * nr;
 
+    if (ADJ && G != 0.f) {  // path-length scores of this segment
+      const float ca = -G * nr * w.lk, cb = -G * w.la;
+      split_layers(P, z, I, sign, anz, edh, del, [&](int k, float l) {
+        atomicAdd(&gacc[k], ca * P.adj_dpda[k] * l);
+        atomicAdd(&gacc[P.size + k], cb * P.adj_ds1db[k] * l);
+      });
+    }
+
     r.x += del * n.x, r.y += del * n.y, r.z += del * n.z, r.w += del * n.w;
 
     if (!isfinite(TOT) || !isfinite(SCA)) TOT = 0, om = -1, n_nonfinite++;
 
-    if (om != -1) {  // DOM hit
+    if (ADJ && om != -1) {
+      if (om == ofla)
+        TOT = 0;
+      else {
+        if (ph.seg < P.adj_nseg) G -= P.adj_weight[(size_t)ph.seg * P.gsize + om];
+        if (P.xR == 1)
+          TOT = 0;
+        else
+          old = om;
+      }
+    } else if (om != -1) {  // DOM hit
       if (om == ofla)
         TOT = 0;
       else {
@@ -729,6 +797,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         if (slot < P.hnum) {
           P.hits[slot] = h;
           P.hit_photon[slot] = ph.idx;
+          if (P.hit_start) P.hit_start[slot] = make_uint4(ph.x0, ph.c0, gid, 0);
         }
         if (P.xR == 1)
           TOT = 0;
@@ -740,6 +809,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
     if (TOT < XXX) {
       TOT = 0;
       live = false;
+      if (ADJ && fabsf(G) > 1e-3f) n_nonfinite++;  // ADJ: the replay missed a recorded hit
     } else {
       if (EXT && P.sum > 0 && fin > 0) {  // birefringence (gpu/pro.cu:757-836)
         float ra, rb;
@@ -827,6 +897,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
           xi = 2 * (1 - fpow(xi, gr));
         }
         xi = fminf(fmaxf(xi, 0.f), 2.f);
+        if (ADJ && G != 0.f && !(EXT && hole))
+          atomicAdd(&gacc[P.size + J], G * w.la * P.adj_ds1db[J] * frcp(w.sca(T, J)));
         const float4 a = T.az[J];
         const float k1 = a.x, k2 = a.y, okz = k1 * k2;
         const bool frame = !(EXT && hole);
@@ -855,7 +927,12 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
     }
   }
 
-  P.rs[gid] = make_uint2(s.x, s.c);
+  if (ADJ) {  // block sums into the FP64 gradient
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * P.size; k += blockDim.x)
+      if (gacc[k] != 0.f) atomicAdd(&P.adj_grad[k], (double)gacc[k]);
+  } else
+    P.rs[gid] = make_uint2(s.x, s.c);
 
   // per-launch counters: one atomic per wave
   constexpr int NC = COUNT ? 8 : 4;
@@ -877,22 +954,35 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
 
 typedef void (*kfun)(const KParams*);
 
-template <bool COUNT>
+template <bool COUNT, bool ADJ>
 static kfun pick(int tilt, bool ext) {
   if (ext) {
-    if (tilt == 1) return propagate<1, true, COUNT>;
-    if (tilt == 2) return propagate<2, true, COUNT>;
-    return propagate<0, true, COUNT>;
+    if (tilt == 1) return propagate<1, true, COUNT, ADJ>;
+    if (tilt == 2) return propagate<2, true, COUNT, ADJ>;
+    return propagate<0, true, COUNT, ADJ>;
   }
-  if (tilt == 1) return propagate<1, false, COUNT>;
-  if (tilt == 2) return propagate<2, false, COUNT>;
-  return propagate<0, false, COUNT>;
+  if (tilt == 1) return propagate<1, false, COUNT, ADJ>;
+  if (tilt == 2) return propagate<2, false, COUNT, ADJ>;
+  return propagate<0, false, COUNT, ADJ>;
+}
+
+__global__ void reset_counters(Counters* ctr) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(ctr);
+  for (uint32_t k = threadIdx.x; k < sizeof(Counters) / 4; k += blockDim.x) w[k] = 0;
+}
+
+// a kernel rather than a memset, so that captured graphs hold no memset nodes
+hipError_t launch_reset(Counters* ctr, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, stream, ctr);
+  return hipGetLastError();
 }
 
 hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int threads, size_t lds_bytes,
-                            hipStream_t stream, bool count) {
+                            hipStream_t stream, bool count, bool adjoint) {
   const bool ext = P.sum > 0 || P.hr > 0 || P.fr != 0;
-  kfun f = count ? pick<true>(P.tmod, ext) : pick<false>(P.tmod, ext);
+  kfun f = adjoint ? pick<false, true>(P.tmod, ext)
+                   : count ? pick<true, false>(P.tmod, ext) : pick<false, false>(P.tmod, ext);
+  if (adjoint) lds_bytes += sizeof(float) * 2 * P.size;
   hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, dP);
   return hipGetLastError();
 }

@@ -56,6 +56,7 @@ const void* xppc_c_field(const char* name, uint64_t* bytes) {
   ARRAY("lay", det.lay) ARRAY("az", det.az) ARRAY("wv", det.wv) ARRAY("is", det.is) ARRAY("ls", det.ls)
   ARRAY("sc", det.sc) ARRAY("lp", det.lp) ARRAY("oms", det.oms) ARRAY("rm", det.rm)
   ARRAY("abs_", det.abs_tab) ARRAY("sca", det.sca_tab)
+  ARRAY("par_a", det.par_a) ARRAY("par_b", det.par_b) ARRAY("dpda", det.dpda) ARRAY("ds1db", det.ds1db)
   if (n == "str" || n == "dom") {
     g_str.resize(det.names.size());
     g_dom.resize(det.names.size());

@@ -114,6 +114,9 @@ struct Detector {
 
   // ---- dense per-wavelength ice tables as the reference stores them ----
   std::vector<float> abs_tab, sca_tab;  // [WNUM][MAXLYS]
+  // per layer (layer 0 deepest): the icemodel.dat absorption and scattering parameters
+  // adust(400) and be(400), and the derivatives of the layer factors p and s1 by them
+  std::vector<float> par_a, par_b, dpda, ds1db;
   int l0 = 0;
 
   // ---- host post-processing ----

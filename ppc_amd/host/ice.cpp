@@ -1191,6 +1191,8 @@ void Detector::build() {
   }
   const float AB0 = A * std::exp(-B / wv0);
   for (int i = 0; i < size; i++) lay[i].p -= arf * AB0 * lay[i].t;
+  par_a.resize(size), par_b.resize(size), dpda.assign(size, Dc), ds1db.assign(size, 1 / (1 - d.g));
+  for (int i = 0; i < size; i++) par_a[i] = ba[size - 1 - i], par_b[i] = be[size - 1 - i];
 
   wv.assign(WNUM, ppc_hip_wv{});
   abs_tab.assign(WNUM * MAXLYS, 0.f);

@@ -92,8 +92,52 @@ def oracle():
         lib.orc_run_xcpu.argtypes = [ctypes.c_void_p, ctypes.POINTER(Bunch), ctypes.c_int, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.POINTER(Rng), ctypes.POINTER(Hit), ctypes.c_uint32,
                                      ctypes.POINTER(Counters)]
+        lib.orc_adjoint_photon.restype = ctypes.c_int
+        lib.orc_adjoint_photon.argtypes = [ctypes.c_void_p, ctypes.POINTER(Bunch), ctypes.c_int, ctypes.c_uint32,
+                                           ctypes.POINTER(Rng), ctypes.POINTER(Adjoint)]
         _oracle = lib
     return _oracle
+
+
+class Adjoint(ctypes.Structure):
+    _fields_ = [("lk", ctypes.POINTER(ctypes.c_float)), ("la", ctypes.POINTER(ctypes.c_float)),
+                ("dpda", ctypes.POINTER(ctypes.c_float)), ("ds1db", ctypes.POINTER(ctypes.c_float)),
+                ("weight", ctypes.POINTER(ctypes.c_float)), ("nseg", ctypes.c_uint32), ("gsize", ctypes.c_uint32),
+                ("ga", ctypes.POINTER(ctypes.c_double)), ("gb", ctypes.POINTER(ctypes.c_double)),
+                ("G", ctypes.c_double)]
+
+
+def oracle_adjoint(tables, bunches, photons, states, weight, lk, la, dpda, ds1db):
+    """Score-function gradient of the oracle for the given photons (index, MWC
+    state (x, c, a)): per-layer sums (ga, gb) in float64 and the hits per photon."""
+    lib = oracle()
+    size = len(dpda)
+    f32 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    lk, la, dpda, ds1db, weight = f32(lk), f32(la), f32(dpda), f32(ds1db), f32(weight)
+    ga = np.zeros(size, dtype=np.float64)
+    gb = np.zeros(size, dtype=np.float64)
+    P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+    A = Adjoint(P(lk, ctypes.c_float), P(la, ctypes.c_float), P(dpda, ctypes.c_float), P(ds1db, ctypes.c_float),
+                P(weight, ctypes.c_float), weight.shape[0], weight.shape[1], P(ga, ctypes.c_double),
+                P(gb, ctypes.c_double), 0.0)
+    arr = (Bunch * len(bunches))(*bunches)
+    nh = np.zeros(len(photons), dtype=np.int32)
+    for k, (i, st) in enumerate(zip(photons, states)):
+        s = Rng(int(st[0]), int(st[1]), int(st[2]))
+        nh[k] = lib.orc_adjoint_photon(tables, arr, len(bunches), int(i), ctypes.byref(s), ctypes.byref(A))
+    return ga, gb, nh
+
+
+def point_bunch(x, y, z, num, t=0.0, seg=0):
+    """an isotropic point source (a flasher-type bunch with ka = -1 and no DOM):
+    every photon starts at (x, y, z) at time t (gpu/pro.cu:355-378)"""
+    b = Bunch()
+    b.r[:] = [x, y, z, t]
+    b.n[:] = [0.0, 0.0, 1.0, 1.0]
+    b.q, b.num, b.type, b.f = seg, num, 1, 0.0
+    b.ka, b.up, b.fldr = -1.0, 0.0, 0.0
+    b.fla, b.ofla = -1, -1
+    return b
 
 
 def ref_available():
