@@ -15,9 +15,16 @@ The JSON line also carries
   cpu_baseline  the reference's own CPU program (oracle/_ref/ppc, single
                 thread per process) on the same ice, one process per host core,
                 on a bounded cascade sample; the oracle restatement
-                ("port") where the reference program is not built.
+                ("port") where the reference program is not built;
+  adjoint       the gradient of a Poisson LLH of 12 flasher events (1e8 photons
+                each, 7-string ice, 100 layers: 200 parameters): forward and
+                adjoint time, their ratio, and the cost of the same gradient by
+                central differences (2 x 200 forward passes);
+  batch         (--batch) 512 sources of mixed types in 8 ice variants on the
+                7-string ice, one captured-graph replay per step, at the launch
+                shape the autotuner picks.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu] [--no-adjoint] [--batch]
 """
 import argparse
 import json
@@ -103,6 +110,78 @@ def gpu_run(args):
     }
 
 
+def adjoint_run(events=12, photons=1e8, seed=6):
+    """C5: forward + adjoint of a Poisson LLH over 12 flasher events on the 7-string ice"""
+    import numpy as np
+
+    from ppc_amd import adjoint, api, configs
+
+    det = api.Detector(configs.ensure("c1"), {"WFLA": "405", "FWID": "10", "DTMN": None})
+    rng = np.random.default_rng(seed)
+    strs, doms = det.array("str", np.int32), det.array("dom", np.int32)
+    pick = rng.choice(len(strs), events, replace=False)
+    bunches = [det.flasher_bunch(int(strs[d]), int(doms[d]), num=int(photons), seg=k) for k, d in enumerate(pick)]
+    sim = adjoint.HitSimulation(det, bunches, max_hits=1 << 25)
+    size = len(sim.par_a)
+    # synthetic data: Poisson counts at an ice with every layer perturbed by up to 5 %
+    mu_d, _ = sim.forward(sim.par_a * rng.uniform(0.95, 1.05, size), sim.par_b * rng.uniform(0.95, 1.05, size))
+    k = rng.poisson(mu_d)
+    sim.forward(sim.par_a, sim.par_b)  # warm-up
+    t0 = time.perf_counter()
+    mu, launch = sim.forward(sim.par_a, sim.par_b)
+    t1 = time.perf_counter()
+    w = np.where(mu > 0, k / np.maximum(mu, 1e-9) - 1, -1.0)
+    ga, gb = sim.backward(launch, w)
+    t2 = time.perf_counter()
+    sim.close()
+    fwd, adj = t1 - t0, t2 - t1
+    P = 2 * size
+    return {"workload": "%d flasher events x %.0e photons, 7-string ice, %d layers, %d parameters"
+                        % (events, photons, size, P),
+            "forward_s": round(fwd, 4), "forward_kernel_ms": round(launch["stats"].device_ms, 2),
+            "adjoint_s": round(adj, 4), "adjoint_kernel_ms": round(sim.last_adjoint["ms"], 2),
+            "replayed_photons": int(sim.last_adjoint["photons"]),
+            "gradient_cost_vs_forward": round((fwd + adj) / fwd, 3),
+            "fd_cost_vs_forward": 2 * P,
+            "gradient_evals_per_s": round(1.0 / (fwd + adj), 3),
+            "fd_gradient_evals_per_s": round(1.0 / (2 * P * fwd), 5),
+            "grad_norm": [float(np.abs(ga).sum()), float(np.abs(gb).sum())]}
+
+
+def batch_run(steps, seed=5):
+    """C4: 512 mixed sources in 8 ice variants, captured graph, autotuned launch shape"""
+    from ppc_amd import api, configs, tune
+
+    det = api.Detector(configs.ensure("c1"), {"WFLA": "405", "FWID": "10", "DTMN": None})
+    bunches, variant, layers = configs.c4_batch(det, seed=seed)
+    num = sum(b.num for b in bunches)
+    best, table = tune.tune(det, bunches, variant, layers, photons=2e8, repeats=1)
+    cus = tune.device_cus()
+    dev = api.Device(0, max_bunches=len(bunches), max_hits=1 << 26, **tune.device_kwargs(best, cus))
+    dev.upload(det)
+    dev.set_layers(layers)
+    dev.graph_capture(bunches, variant, num)
+    dev.graph_launch()
+    dev.wait()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.graph_launch()
+        st = dev.wait()
+        ms.append(st.device_ms)
+        assert st.photons == num
+    wall = time.perf_counter() - t0
+    dev.close()
+    return {"workload": "512 sources (flashers, cascades, 10 m muon segments), photons log-uniform in [1e5, 1e8], "
+                        "8 ice variants of the 7-string ice", "photons_per_step": num,
+            "photons_per_s": round(num * steps / wall, 1), "graph_ms": round(sum(ms) / len(ms), 2),
+            "tuned_shape": best, "candidates": len(table),
+            "untuned_default_ms": next((round(t, 2) for sh, t in table
+                                        if sh == {"threads": 256, "waves_per_cu": 16, "refill": 16, "chunk": 0,
+                                                  "tilt_lds": 0}), None),
+            "tuned_ms_at_2e8": round(min(t for _, t in table), 2)}
+
+
 def cpu_reference(cores, seconds):
     """the reference's CPU program, one process per core, cascade photons from stdin"""
     from ppc_amd import configs
@@ -184,6 +263,8 @@ def main():
     ap.add_argument("--photons", type=float, default=1e9)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagator runs on a single MI355X (one photon stream per lane, "
@@ -192,6 +273,8 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
     g = gpu_run(args)
+    adj = None if args.no_adjoint else adjoint_run()
+    bat = batch_run(args.steps) if args.batch else None
     cpu = None
     if not args.no_cpu:
         cores = max(1, min(16, os.cpu_count() or 1))
@@ -220,6 +303,10 @@ def main():
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
     }
+    if adj:
+        out["adjoint"] = adj
+    if bat:
+        out["batch"] = bat
     if cpu:
         out["config"]["gpu_over_cpu"] = round(g["value"] / cpu["value"], 1)
     print(json.dumps(out))

@@ -155,7 +155,9 @@ typedef struct {
   uint32_t rec_stride;     /* record the MWC state at emission of every rec_stride-th photon (0: off) */
   uint32_t rec_max;        /* capacity of the record buffer */
   int32_t count_events;    /* 1: also count layer crossings, string and DOM tests, scatterings (slower) */
-  int32_t record_starts;   /* 1: keep, per stored hit, the MWC state its photon started from (adjoint) */
+  int32_t record_starts;   /* 1: record the start state of every photon that hits (adjoint), max_hits of them */
+  uint32_t chunk;          /* photons a wave claims per atomic on the work counter; 0: num/(16 waves), 64..1024 */
+  int32_t tilt_lds;        /* 1-d/2-d tilt table: 0 in LDS when it fits in 16 KB, 1 in LDS when it fits, -1 in HBM */
 } ppc_hip_config;
 
 /* per-launch results (what gpu/ppc.cu:255-283 reads back) */
@@ -173,6 +175,7 @@ typedef struct {
   uint64_t strings;   /* string (cylinder) intersection tests */
   uint64_t doms;      /* DOM (ellipsoid) intersection tests */
   uint64_t scatters;  /* scatterings */
+  uint64_t hit_photons; /* photons that hit (record_starts; records beyond max_hits are dropped) */
 } ppc_hip_stats;
 
 /* replay record: MWC state of a lane when it starts emitting photon `photon` */
@@ -181,19 +184,11 @@ typedef struct {
   uint32_t x, c, a;
 } ppc_hip_record;
 
-/* start of the photon of a stored hit (record_starts): MWC state and lane */
-typedef struct {
-  uint32_t x, c, lane, pad;
-} ppc_hip_start;
-
-/* one photon to replay in an adjoint pass: its index in the forward launch, the
- * MWC state and lane it started from, and the sum over its hits of the LLH
- * weight of the hit DOM */
+/* a photon that hit (record_starts), to be replayed by the adjoint: its index in
+ * the launch and the MWC state and lane it started from */
 typedef struct {
   uint32_t photon;
   uint32_t x0, c0, lane;
-  float weight;
-  uint32_t pad[3];
 } ppc_hip_adj_record;
 
 typedef struct ppc_hip_ctx ppc_hip_ctx;
@@ -249,15 +244,19 @@ int ppc_hip_graph_capture(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, const 
 int ppc_hip_graph_launch(ppc_hip_ctx* ctx);
 
 /* ---- gradient of a Poisson log-likelihood of the hit counts ---- */
-/* the start states of the stored hits of the last launch (record_starts) */
-int ppc_hip_hit_starts(ppc_hip_ctx* ctx, ppc_hip_start* starts, uint32_t cap, uint32_t* n);
-/* Replay the recorded photons of a forward launch of these bunches (same num,
- * ice variant 0) and accumulate into grad[0..size) d(LLH)/d(absorption
- * parameter a_i) and into grad[size..2*size) d(LLH)/d(scattering parameter
- * b_i).  weight[s*gsize + dom] is d(LLH)/d(mean hits of dom) for track segment
- * s < nseg; dpda[i] = d(p_i)/d(a_i) and ds1db[i] = d(s1_i)/d(b_i) relate the
- * parameters to the factorised layer table.  stats->nonfinite counts replayed
- * photons whose hits did not reproduce the recorded ones.  Synchronous. */
+/* the photons of the last launch that hit (record_starts), in no particular order */
+int ppc_hip_hit_photons(ppc_hip_ctx* ctx, ppc_hip_adj_record* rec, uint32_t cap, uint32_t* n);
+/* Replay photons of a forward launch of these bunches (same bunches and num, ice
+ * variant 0) and accumulate into grad[0..size) d(LLH)/d(absorption parameter
+ * a_i) and into grad[size..2*size) d(LLH)/d(scattering parameter b_i).  The
+ * photons are rec[0..nrec), or with rec == NULL the hitting photons recorded by
+ * the last launch (record_starts; it must have been waited for).
+ * weight[s*gsize + dom] is d(LLH)/d(mean hits of dom) for track segment s < nseg
+ * (hits of later segments weigh 0); dpda[i] = d(p_i)/d(a_i) and
+ * ds1db[i] = d(s1_i)/d(b_i) relate the parameters to the factorised layer table.
+ * Each photon runs twice, summing its hits' weights and then scoring its path;
+ * stats->photons counts both runs and stats->nonfinite the photons whose two runs
+ * disagreed.  The results of the forward launch stay readable.  Synchronous. */
 int ppc_hip_adjoint(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
                     const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
                     const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* stats);

@@ -13,7 +13,8 @@ every straight segment of length l_k in layer k the absorption score is
 -nr*lk*dp/da*l_k and the scattering score -la*ds1/db*l_k, plus
 la*ds1/db / sca_J at every scattering in layer J.  The backward pass replays
 only the photons that hit, each from the MWC state it started from (kept by the
-forward launch), with the weight of the hits it still has to make; it runs on
+forward launch on the device): once to sum the weights of its hits, once to
+score its path with the weight of the hits it still has to make.  It runs on
 the GPU (ppc_hip_adjoint) and accumulates in FP64.
 
     sim = HitSimulation(det, bunches, scale=1.0)
@@ -26,34 +27,6 @@ import numpy as np
 import torch
 
 from . import api
-
-
-def adjoint_records(hits, photon, starts, weight):
-    """One adjoint record per photon with a weighted hit: its start state and the
-    summed weight[seg, dom] of its stored hits (hits of segments beyond
-    weight's first axis weigh 0)."""
-    nseg = weight.shape[0]
-    seg = hits["n"].astype(np.int64)
-    ok = seg < nseg
-    w = np.zeros(len(hits), dtype=np.float64)
-    w[ok] = weight[seg[ok], hits["i"][ok]]
-    rec = np.zeros(0, dtype=api.ADJ_RECORD_DTYPE)
-    if len(hits) == 0:
-        return rec
-    order = np.argsort(photon, kind="stable")
-    ph = photon[order]
-    first = np.flatnonzero(np.r_[True, ph[1:] != ph[:-1]])
-    wsum = np.add.reduceat(w[order], first)
-    wabs = np.add.reduceat(np.abs(w[order]), first)
-    keep = wabs > 0
-    sel = order[first][keep]
-    rec = np.zeros(int(keep.sum()), dtype=api.ADJ_RECORD_DTYPE)
-    rec["photon"] = photon[sel]
-    rec["x0"] = starts["x"][sel]
-    rec["c0"] = starts["c"][sel]
-    rec["lane"] = starts["lane"][sel]
-    rec["weight"] = wsum[keep]
-    return rec
 
 
 class HitSimulation:
@@ -82,23 +55,27 @@ class HitSimulation:
         self.dev.set_layers(self.det.layers_for(a, b))
         self.dev.launch(self.bunches, self.num)
         st = self.dev.wait()
-        if st.hits > st.stored:
-            raise api.PPCError("%d hits exceed the hit buffer (%d); open the device with a larger max_hits"
-                               % (st.hits, st.stored))
-        hits, photon = self.dev.hits(with_photon=True)
-        starts = self.dev.hit_starts()
+        if st.hits > st.stored or st.hit_photons > self.dev.max_hits:
+            raise api.PPCError("%d hits of %d photons exceed the buffers (%d); open the device with a larger "
+                               "max_hits" % (st.hits, st.hit_photons, self.dev.max_hits))
+        hits = self.dev.hits()
         seg = hits["n"].astype(np.int64)
         ok = seg < self.nseg
         n = np.zeros((self.nseg, self.gsize), dtype=np.float64)
         np.add.at(n, (seg[ok], hits["i"][ok].astype(np.int64)), 1.0)
-        return n * self.scale, {"hits": hits, "photon": photon, "starts": starts, "stats": st}
+        self._launches = getattr(self, "_launches", 0) + 1
+        return n * self.scale, {"id": self._launches, "hits": hits, "stats": st}
 
     def backward(self, launch, grad_mu):
-        """d(L)/d(a), d(L)/d(b) given dL/dmu [nseg, gsize] of the launch's mu"""
+        """d(L)/d(a), d(L)/d(b) given dL/dmu [nseg, gsize] of the launch's mu.  The
+        adjoint replays the hitting photons the launch recorded on the device, so
+        it must follow its own forward launch."""
+        if launch["id"] != self._launches:
+            raise api.PPCError("the adjoint replays the most recent forward launch only")
         weight = np.asarray(grad_mu, dtype=np.float64) * self.scale
-        rec = adjoint_records(launch["hits"], launch["photon"], launch["starts"], weight)
-        ga, gb, st = self.dev.adjoint(self.bunches, self.num, rec, weight.astype(np.float32), self.dpda, self.ds1db)
-        self.last_adjoint = {"records": len(rec), "missed": st.nonfinite, "ms": st.device_ms}
+        ga, gb, st = self.dev.adjoint(self.bunches, self.num, weight.astype(np.float32), self.dpda, self.ds1db)
+        self.last_adjoint = {"photons": launch["stats"].hit_photons, "replays": st.photons,
+                             "missed": st.nonfinite, "ms": st.device_ms}
         return ga, gb
 
 

@@ -44,9 +44,7 @@ class Hit(ctypes.Structure):
 HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"), ("pph", "<f4"),
                       ("dth", "<f4"), ("dph", "<f4")])
 RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<u4")])
-START_DTYPE = np.dtype([("x", "<u4"), ("c", "<u4"), ("lane", "<u4"), ("pad", "<u4")])
-ADJ_RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x0", "<u4"), ("c0", "<u4"), ("lane", "<u4"), ("weight", "<f4"),
-                             ("pad", "<u4", (3,))])
+ADJ_RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x0", "<u4"), ("c0", "<u4"), ("lane", "<u4")])
 LAYER_DTYPE = np.dtype([("s0", "<f4"), ("s1", "<f4"), ("p", "<f4"), ("t", "<f4")])
 
 
@@ -54,14 +52,15 @@ class Config(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32), ("refill", ctypes.c_int32),
                 ("deterministic", ctypes.c_int32), ("max_bunches", ctypes.c_uint32), ("max_hits", ctypes.c_uint32),
                 ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32), ("count_events", ctypes.c_int32),
-                ("record_starts", ctypes.c_int32)]
+                ("record_starts", ctypes.c_int32), ("chunk", ctypes.c_uint32), ("tilt_lds", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("photons", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("hits", ctypes.c_uint64),
                 ("stored", ctypes.c_uint64), ("killed", ctypes.c_uint64), ("nonfinite", ctypes.c_uint64),
                 ("device_ms", ctypes.c_float), ("lanes", ctypes.c_int32), ("crossings", ctypes.c_uint64),
-                ("strings", ctypes.c_uint64), ("doms", ctypes.c_uint64), ("scatters", ctypes.c_uint64)]
+                ("strings", ctypes.c_uint64), ("doms", ctypes.c_uint64), ("scatters", ctypes.c_uint64),
+                ("hit_photons", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -100,7 +99,7 @@ def hip():
         lib.ppc_hip_launch_batch.argtypes = [c_p, ctypes.POINTER(Bunch), c_p, ctypes.c_uint32, ctypes.c_uint32]
         lib.ppc_hip_graph_capture.argtypes = [c_p, ctypes.POINTER(Bunch), c_p, ctypes.c_uint32, ctypes.c_uint32]
         lib.ppc_hip_graph_launch.argtypes = [c_p]
-        lib.ppc_hip_hit_starts.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_hit_photons.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         lib.ppc_hip_adjoint.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32, c_p,
                                         ctypes.c_uint32, c_p, ctypes.c_uint32, c_p, c_p, c_p, ctypes.POINTER(Stats)]
         _hip = lib
@@ -247,10 +246,10 @@ class Device:
     """One GPU context of libppc_hip.so."""
 
     def __init__(self, device=0, blocks=0, threads=0, refill=0, deterministic=False, max_bunches=0, max_hits=0,
-                 rec_stride=0, rec_max=0, count_events=False, record_starts=False):
+                 rec_stride=0, rec_max=0, count_events=False, record_starts=False, chunk=0, tilt_lds=0):
         self.lib = hip()
         self.cfg = Config(blocks, threads, refill, 1 if deterministic else 0, max_bunches, max_hits, rec_stride,
-                          rec_max, 1 if count_events else 0, 1 if record_starts else 0)
+                          rec_max, 1 if count_events else 0, 1 if record_starts else 0, chunk, tilt_lds)
         self.ctx = ctypes.c_void_p()
         _check(self.lib.ppc_hip_open(device, ctypes.byref(self.cfg), ctypes.byref(self.ctx)))
         n = ctypes.c_int32()
@@ -363,20 +362,22 @@ class Device:
         _check(self.lib.ppc_hip_graph_launch(self.ctx))
 
     # ---- adjoint ----
-    def hit_starts(self):
+    def hit_photons(self):
+        """the photons of the last launch that hit (record_starts): ADJ_RECORD_DTYPE"""
         st = Stats()
         _check(self.lib.ppc_hip_wait(self.ctx, ctypes.byref(st)))
-        m = int(st.stored)
-        a = np.zeros(m, dtype=START_DTYPE)
+        m = int(min(st.hit_photons, self.max_hits))
+        a = np.zeros(m, dtype=ADJ_RECORD_DTYPE)
         n = ctypes.c_uint32()
-        _check(self.lib.ppc_hip_hit_starts(self.ctx, a.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
+        _check(self.lib.ppc_hip_hit_photons(self.ctx, a.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
         return a[:n.value]
 
-    def adjoint(self, bunches, num, records, weight, dpda, ds1db):
-        """replay records (ADJ_RECORD_DTYPE) of a forward launch of these bunches and
-        return (d/d adust[size], d/d be[size], stats); weight [nseg][gsize] float32"""
+    def adjoint(self, bunches, num, weight, dpda, ds1db, records=None):
+        """replay photons of a forward launch of these bunches -- ``records``
+        (ADJ_RECORD_DTYPE), or by default the hitting photons the last launch
+        recorded -- and return (d/d adust[size], d/d be[size], stats);
+        weight [nseg][gsize] is d(LLH)/d(mean hits)"""
         arr, _ = self._bunches(bunches, None)
-        records = np.ascontiguousarray(records, dtype=ADJ_RECORD_DTYPE)
         weight = np.ascontiguousarray(weight, dtype=np.float32)
         nseg = weight.shape[0] if weight.ndim == 2 else 1
         dpda = np.ascontiguousarray(dpda, dtype=np.float32)
@@ -384,8 +385,13 @@ class Device:
         size = len(dpda)
         grad = np.zeros(2 * size, dtype=np.float64)
         st = Stats()
-        _check(self.lib.ppc_hip_adjoint(self.ctx, arr, len(arr), num, records.ctypes.data_as(ctypes.c_void_p),
-                                        len(records), weight.ctypes.data_as(ctypes.c_void_p), nseg,
+        if records is not None:
+            records = np.ascontiguousarray(records, dtype=ADJ_RECORD_DTYPE)
+            rp, nrec = records.ctypes.data_as(ctypes.c_void_p), len(records)
+        else:
+            rp, nrec = None, 0
+        _check(self.lib.ppc_hip_adjoint(self.ctx, arr, len(arr), num, rp, nrec,
+                                        weight.ctypes.data_as(ctypes.c_void_p), nseg,
                                         dpda.ctypes.data_as(ctypes.c_void_p), ds1db.ctypes.data_as(ctypes.c_void_p),
                                         grad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
         return grad[:size], grad[size:], st

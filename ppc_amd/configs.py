@@ -340,6 +340,46 @@ def c3_muon_event(seed=4, length=1200.0, step=1.0, E=1000.0):
     return recs
 
 
+def c4_batch(det, seed=5, nsrc=512, nvar=8, lo=1e5, hi=1e8, scale=1.0):
+    """The batched workload on the 7-string ice (det must hold it): nsrc sources,
+    a third each flashers at random DOMs, EM cascades and 10 m muon segments at
+    random points of the array with random directions, photon counts log-uniform
+    in [lo, hi] (times scale), in nvar ice variants whose per-layer adust(400)
+    and be(400) are the loaded ice's scaled by factors drawn in [0.9, 1.1].
+
+    Returns (bunches, variant per bunch, layer factors [nvar][size])."""
+    rng = np.random.default_rng(seed)
+    q = det.ice_parameters()
+    size = len(q["par_a"])
+    layers = np.stack([det.layers_for(q["par_a"] * rng.uniform(0.9, 1.1, size),
+                                      q["par_b"] * rng.uniform(0.9, 1.1, size)) for _ in range(nvar)])
+    strs = det.array("str", np.int32)
+    doms = det.array("dom", np.int32)
+    oms = det.array("oms", np.float32).reshape(-1, 5)
+    bunches, variant = [], []
+    for k in range(nsrc):
+        num = max(1, int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))) * scale)))
+        kind = k % 3
+        if kind == 0:
+            d = int(rng.integers(len(strs)))
+            b = det.flasher_bunch(int(strs[d]), int(doms[d]), num=num, seg=k)
+        else:
+            d = oms[int(rng.integers(len(oms)))]
+            x, y, z = float(d[2] + rng.uniform(-60, 60)), float(d[3] + rng.uniform(-60, 60)), float(d[4])
+            ct = rng.uniform(-1, 1)
+            st = math.sqrt(1 - ct * ct)
+            phi = rng.uniform(0, 2 * math.pi)
+            n = (st * math.cos(phi), st * math.sin(phi), ct)
+            if kind == 1:
+                b, _ = det.particle_bunch(x, y, z, 0.0, float(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))),
+                                          0.0, 11, n, num=num, seg=k)
+            else:
+                b, _ = det.particle_bunch(x, y, z, 0.0, 1e3, 10.0, 0, n, num=num, seg=k)
+        bunches.append(b)
+        variant.append(int(rng.integers(nvar)))
+    return bunches, np.array(variant, dtype=np.uint16), layers
+
+
 def f2k_lines(recs):
     """the records as f2k TR lines"""
     return ["TR 1 0 %s %.4f %.4f %.4f %.5f %.5f %.4f %.6g %.4f" % (r["name"], r["x"], r["y"], r["z"], r["th"], r["ph"],

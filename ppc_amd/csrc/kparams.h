@@ -25,7 +25,7 @@ struct Counters {
   unsigned long long crossings, strings, doms, scatters;
   unsigned int next;   // next photon to hand out (dynamic refill)
   unsigned int hidx;   // next free hit slot
-  unsigned int nrec;   // records written
+  unsigned int nphot;  // photons with hits recorded for the adjoint
   unsigned int pad[3];
 };
 
@@ -79,7 +79,8 @@ struct KParams {
   int refill, deterministic;
   uint32_t chunk;  // photons claimed per atomic on the global counter
   const uint16_t* bvar;  // ice variant of each bunch (nullptr: variant 0)
-  uint4* hit_start;      // per hit: MWC state before emission (x, c) and lane, or nullptr
+  ppc_hip_adj_record* hit_photons;  // start state of every photon that hits, or nullptr
+  uint32_t hit_photons_cap;
   // adjoint replay
   const ppc_hip_adj_record* adj_rec;
   uint32_t adj_nrec, adj_nseg;

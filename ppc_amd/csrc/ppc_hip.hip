@@ -61,6 +61,7 @@ struct ppc_hip_ctx {
   int lanes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
+  hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr;  // adjoint kernel
   bool launched = false, uploaded = false, copy_pending = false;
 
   // launch buffers
@@ -92,7 +93,11 @@ struct ppc_hip_ctx {
   int nvar = 1;                   // ice variants in the layer table
   uint16_t* d_bvar = nullptr;     // per-bunch ice variant of a batch launch
   uint16_t* h_bvar = nullptr;
-  uint4* d_hit_start = nullptr;   // per-hit start state (record_starts)
+  ppc_hip_adj_record* d_hphot = nullptr;  // photons that hit (record_starts)
+  uint32_t fwd_nphot = 0;         // their number in the last waited-for launch
+  bool fwd_waited = false;
+  ppc::Counters* d_ctr_adj = nullptr;  // the adjoint's own counters
+  ppc::Counters* h_ctr_adj = nullptr;
   // adjoint
   ppc_hip_adj_record* d_adj_rec = nullptr;
   uint32_t adj_cap = 0;
@@ -187,6 +192,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   // 16 waves per CU (4 per SIMD: the kernel's register budget allows no more)
   if (k.blocks <= 0) k.blocks = c->cus * (1024 / k.threads);
   if (k.refill <= 0) k.refill = 16;
+  if (k.chunk > 1024) return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);
   if (k.refill > 64) k.refill = 64;
   if (k.max_bunches == 0) k.max_bunches = 1u << 20;
   if (k.max_hits == 0) k.max_hits = 1u << 24;
@@ -200,6 +206,8 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(hipEventCreate(&c->ev0));
   chk(hipEventCreate(&c->ev1));
   chk(hipEventCreateWithFlags(&c->ev_copy, hipEventDisableTiming));
+  chk(hipEventCreate(&c->ev_a0));
+  chk(hipEventCreate(&c->ev_a1));
   chk(dalloc(&c->d_pz, k.max_bunches));
   chk(dalloc(&c->d_pstart, k.max_bunches + 1));
   chk(dalloc(&c->d_hits, k.max_hits));
@@ -215,7 +223,9 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(dalloc(&c->d_P, 1));
   chk(dalloc(&c->d_bvar, k.max_bunches));
   chk(hipHostMalloc((void**)&c->h_bvar, sizeof(uint16_t) * k.max_bunches, hipHostMallocDefault));
-  if (k.record_starts) chk(dalloc(&c->d_hit_start, k.max_hits));
+  if (k.record_starts) chk(dalloc(&c->d_hphot, k.max_hits));
+  chk(dalloc(&c->d_ctr_adj, 1));
+  chk(hipHostMalloc((void**)&c->h_ctr_adj, sizeof(ppc::Counters), hipHostMallocDefault));
   if (e != hipSuccess) {
     int code = e == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE;
     ppc_hip_close(c);
@@ -231,15 +241,17 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hit_start,
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj,
                   (void*)c->d_adj_rec, (void*)c->d_adj_w, (void*)c->d_adj_dp, (void*)c->d_adj_grad})
     if (p) (void)hipFree(p);
   drop_graph(c);
-  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr, (void*)c->h_P, (void*)c->h_bvar})
+  for (void* p : {(void*)c->h_pz, (void*)c->h_pstart, (void*)c->h_ctr, (void*)c->h_P, (void*)c->h_bvar, (void*)c->h_ctr_adj})
     if (p) (void)hipHostFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev_copy) (void)hipEventDestroy(c->ev_copy);
+  if (c->ev_a0) (void)hipEventDestroy(c->ev_a0);
+  if (c->ev_a1) (void)hipEventDestroy(c->ev_a1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PPC_HIP_OK;
@@ -293,6 +305,7 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   }
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graph(c);  // a captured graph reads the tables it was captured with
 
   KParams& P = c->P;
   P = KParams{};
@@ -356,7 +369,11 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     for (int k = 0; k < lp_stride; k++)
       lp[i * lp_stride + k] = make_float2(t->lp[(i * PPC_HIP_LYRS + k) * 2], t->lp[(i * PPC_HIP_LYRS + k) * 2 + 1]);
   P.lp_stride = lp_stride;
-  P.lp_in_lds = lp.size() * sizeof(float2) <= 16384 ? 1 : 0;
+  {
+    const size_t lp_bytes = lp.size() * sizeof(float2);
+    const int mode = c->cfg.tilt_lds;
+    P.lp_in_lds = mode < 0 ? 0 : lp_bytes <= (mode > 0 ? size_t(40960) : size_t(16384)) ? 1 : 0;
+  }
   if (P.lp_in_lds) P.off_lp = put(lp.data(), sizeof(float2) * lp.size());
   img.resize((img.size() + 15) & ~size_t(15));
   c->img_base = img;  // the layer table (ice variants) is appended by set_layers
@@ -408,7 +425,8 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.refill = c->cfg.refill;
   P.deterministic = c->cfg.deterministic;
   P.lanes = (uint32_t)c->lanes;
-  P.hit_start = c->d_hit_start;
+  P.hit_photons = c->d_hphot;
+  P.hit_photons_cap = c->d_hphot ? c->cfg.max_hits : 0;
   c->uploaded = true;
   return ppc_hip_seed(c, 0);
 }
@@ -471,7 +489,7 @@ int stage(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant,
      // enough that the last chunks of a launch still spread over all waves
     const uint32_t waves = (uint32_t)c->lanes / 64;
     uint64_t ch = num / ((uint64_t)waves * 16 + 1);
-    c->h_P->chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ch, 64), 1024);
+    c->h_P->chunk = c->cfg.chunk ? c->cfg.chunk : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ch, 64), 1024);
   }
   return PPC_HIP_OK;
 }
@@ -510,6 +528,7 @@ int ppc_hip_launch_batch(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uin
   if (rc != PPC_HIP_OK) return rc;
   c->copy_pending = true;
   c->launched = true;
+  c->fwd_waited = false;
   c->last_num = num;
   return PPC_HIP_OK;
 }
@@ -572,19 +591,20 @@ int ppc_hip_graph_launch(ppc_hip_ctx* c) {
   HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->launched = true;
+  c->fwd_waited = false;
   c->last_num = c->graph_num;
   return PPC_HIP_OK;
 }
 
-int ppc_hip_hit_starts(ppc_hip_ctx* c, ppc_hip_start* starts, uint32_t cap, uint32_t* n) {
+int ppc_hip_hit_photons(ppc_hip_ctx* c, ppc_hip_adj_record* rec, uint32_t cap, uint32_t* n) {
   if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
-  if (!c->d_hit_start) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
-  if (!c->launched) return fail(PPC_HIP_E_STATE, "hit starts without launch");
+  if (!c->d_hphot) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "hit photons without launch");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  uint32_t m = std::min<uint32_t>(std::min<uint32_t>(c->h_ctr->hidx, c->cfg.max_hits), cap);
-  if (m > 0 && starts) HIPCHK(hipMemcpy(starts, c->d_hit_start, sizeof(ppc_hip_start) * m, hipMemcpyDeviceToHost));
+  uint32_t m = std::min<uint32_t>(std::min<uint32_t>(c->h_ctr->nphot, c->cfg.max_hits), cap);
+  if (m > 0 && rec) HIPCHK(hipMemcpy(rec, c->d_hphot, sizeof(ppc_hip_adj_record) * m, hipMemcpyDeviceToHost));
   *n = m;
   return PPC_HIP_OK;
 }
@@ -593,14 +613,23 @@ int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, u
                     const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
                     const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* st) {
   if (!c || !grad || !dpda || !ds1db || (nrec && !rec) || (nseg && !weight)) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (!c->uploaded) return fail(PPC_HIP_E_STATE, "adjoint before upload");
   if (c->P.hr > 0) return fail(PPC_HIP_E_ARG, "the adjoint does not model hole ice");
-  for (uint32_t k = 0; k < nrec; k++)
-    if (rec[k].lane >= (uint32_t)c->lanes || rec[k].photon >= num)
-      return fail(PPC_HIP_E_ARG, "adjoint record %u: photon %u lane %u out of range", k, rec[k].photon, rec[k].lane);
+  if (!rec) {  // the hitting photons the last launch recorded
+    if (!c->d_hphot) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
+    if (!c->fwd_waited) return fail(PPC_HIP_E_STATE, "wait for the forward launch before its adjoint");
+    if (num != c->last_num) return fail(PPC_HIP_E_ARG, "adjoint of %u photons, the launch had %u", num, c->last_num);
+    nrec = c->fwd_nphot;
+  } else {
+    for (uint32_t k = 0; k < nrec; k++)
+      if (rec[k].lane >= (uint32_t)c->lanes || rec[k].photon >= num)
+        return fail(PPC_HIP_E_ARG, "adjoint record %u: photon %u lane %u out of range", k, rec[k].photon,
+                    rec[k].lane);
+  }
   int rc = stage(c, bunches, nullptr, nb, num);
   if (rc != PPC_HIP_OK) return rc;
   const int size = c->P.size;
-  if (nrec > c->adj_cap) {
+  if (rec && nrec > c->adj_cap) {
     if (c->d_adj_rec) (void)hipFree(c->d_adj_rec);
     c->d_adj_rec = nullptr;
     HIPCHK(dalloc(&c->d_adj_rec, nrec));
@@ -615,36 +644,38 @@ int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, u
   }
   if (!c->d_adj_dp) HIPCHK(dalloc(&c->d_adj_dp, 2 * PPC_HIP_MAXLYS));
   if (!c->d_adj_grad) HIPCHK(dalloc(&c->d_adj_grad, 2 * PPC_HIP_MAXLYS));
-  if (nrec) HIPCHK(hipMemcpy(c->d_adj_rec, rec, sizeof(ppc_hip_adj_record) * nrec, hipMemcpyHostToDevice));
+  // the stream is idle here (stage waited for the staging copy; synchronous calls below)
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rec && nrec) HIPCHK(hipMemcpy(c->d_adj_rec, rec, sizeof(ppc_hip_adj_record) * nrec, hipMemcpyHostToDevice));
   if (nseg) HIPCHK(hipMemcpy(c->d_adj_w, weight, sizeof(float) * nseg * c->P.gsize, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_adj_dp, dpda, sizeof(float) * size, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_adj_dp + size, ds1db, sizeof(float) * size, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->d_adj_grad, 0, sizeof(double) * 2 * size));
   KParams& H = *c->h_P;
-  H.adj_rec = c->d_adj_rec, H.adj_nrec = nrec, H.adj_nseg = nseg;
+  H.adj_rec = rec ? c->d_adj_rec : c->d_hphot, H.adj_nrec = nrec, H.adj_nseg = nseg;
   H.adj_weight = c->d_adj_w, H.adj_dpda = c->d_adj_dp, H.adj_ds1db = c->d_adj_dp + size;
   H.adj_grad = c->d_adj_grad;
-  H.hit_start = nullptr;
+  H.ctr = c->d_ctr_adj;     // the forward launch's counters and hits stay as they are
+  H.hit_photons = nullptr;
   HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(ppc::launch_reset(c->d_ctr, c->stream));
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  HIPCHK(ppc::launch_reset(c->d_ctr_adj, c->stream));
+  HIPCHK(hipEventRecord(c->ev_a0, c->stream));
   if (nrec)
     HIPCHK(ppc::launch_propagate(H, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, false, true));
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipEventRecord(c->ev_a1, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_ctr_adj, c->d_ctr_adj, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(grad, c->d_adj_grad, sizeof(double) * 2 * size, hipMemcpyDeviceToHost));
   if (st) {
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    const ppc::Counters& k = *c->h_ctr;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1));
+    const ppc::Counters& k = *c->h_ctr_adj;
     *st = ppc_hip_stats{};
     st->photons = k.photons, st->steps = k.steps, st->killed = k.killed, st->nonfinite = k.nonfinite;
     st->device_ms = ms, st->lanes = c->lanes;
   }
-  c->launched = false;  // the hit buffer belongs to the last forward launch, which it no longer matches
   return PPC_HIP_OK;
 }
 
@@ -667,7 +698,10 @@ int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {
     st->device_ms = ms;
     st->lanes = c->lanes;
     st->crossings = k.crossings, st->strings = k.strings, st->doms = k.doms, st->scatters = k.scatters;
+    st->hit_photons = k.nphot;
   }
+  c->fwd_nphot = c->d_hphot ? std::min<uint32_t>(c->h_ctr->nphot, c->cfg.max_hits) : 0;
+  c->fwd_waited = true;
   return PPC_HIP_OK;
 }
 

@@ -278,6 +278,7 @@ struct Photon {
   int old, ofla;
   uint32_t seg, wbin, idx;
   uint32_t x0, c0;  // MWC state before emission (replay records)
+  uint32_t nh;      // hits of this photon so far
   Wave w;
 };
 
@@ -471,7 +472,12 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
   const uint32_t chunk = P.chunk;
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;  // event counts (COUNT builds only)
-  float G = 0.f;  // ADJ: LLH weight of the hits the photon has still to make
+  // ADJ: each record's photon runs twice from its start state: pass 0 sums the LLH
+  // weights of its hits (Gsum), pass 1 scores the path with G, the weight of the hits
+  // still to come
+  float G = 0.f, Gsum = 0.f;
+  uint32_t nw = 0;  // weighted hits of the photon: counted in pass 0, counted down in pass 1
+  bool pass1 = false, pend = false;
   const uint32_t nwork = ADJ ? P.adj_nrec : P.num;
 
   while (true) {
@@ -482,17 +488,21 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       if (nidle >= P.refill || nidle == __popcll(__ballot(true))) {
         uint32_t i = 0xffffffffu;
         if (ADJ) {  // one record after the other per lane
-          if (!live) {
+          if (!live && pend) {  // the scoring pass of the photon just summed
+            s.x = ph.x0, s.c = ph.c0;
+            i = ph.idx;
+            G = Gsum, pass1 = true, pend = false;  // nw carries over from pass 0
+          } else if (!live) {
             const uint32_t k = next_static;
             next_static += P.lanes;
             if (k < nwork) {
               const ppc_hip_adj_record rc = P.adj_rec[k];
               s.x = rc.x0, s.c = rc.c0, s.a = P.rm[rc.lane];
-              G = rc.weight;
               i = rc.photon;
+              G = 0.f, Gsum = 0.f, nw = 0, pass1 = false;
             }
           }
-          done = !__any(next_static < nwork);
+          done = !__any(next_static < nwork || pend);
         } else if (P.deterministic) {
           if (!live) i = next_static, next_static += P.lanes;
           done = !__any(next_static < P.num);
@@ -527,6 +537,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
             P.rec[i / P.rec_stride] = rc;
           }
           ph.idx = i;
+          ph.nh = 0;
           int fla = emit(P, i, s, ph);
           n_photons++;
           ph.TOT = -flog(s());
@@ -763,7 +774,13 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       if (om == ofla)
         TOT = 0;
       else {
-        if (ph.seg < P.adj_nseg) G -= P.adj_weight[(size_t)ph.seg * P.gsize + om];
+        if (ph.seg < P.adj_nseg) {
+          const float wh = P.adj_weight[(size_t)ph.seg * P.gsize + om];
+          if (!pass1)
+            Gsum += wh, nw++;
+          else if (nw > 0)
+            G = --nw == 0 ? 0.f : G - wh;  // exactly 0 after the last weighted hit
+        }
         if (P.xR == 1)
           TOT = 0;
         else
@@ -797,8 +814,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         if (slot < P.hnum) {
           P.hits[slot] = h;
           P.hit_photon[slot] = ph.idx;
-          if (P.hit_start) P.hit_start[slot] = make_uint4(ph.x0, ph.c0, gid, 0);
         }
+        ph.nh++;
         if (P.xR == 1)
           TOT = 0;
         else
@@ -809,7 +826,22 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
     if (TOT < XXX) {
       TOT = 0;
       live = false;
-      if (ADJ && fabsf(G) > 1e-3f) n_nonfinite++;  // ADJ: the replay missed a recorded hit
+      if (ADJ) {
+        if (!pass1 && nw > 0) pend = true;
+        if (pass1 && nw != 0) n_nonfinite++;  // the scoring pass lost a hit of the summing pass
+      } else if (P.hit_photons && ph.nh) {  // start state of a photon that hit, for the adjoint
+        const uint64_t em = __ballot(true);
+        const int leader = __ffsll((unsigned long long)em) - 1;
+        uint32_t first = 0;
+        if ((int)lane == leader) first = atomicAdd(&P.ctr->nphot, (uint32_t)__popcll(em));
+        first = __shfl(first, leader);
+        const uint32_t slot = first + __popcll(em & ((1ull << lane) - 1));
+        if (slot < P.hit_photons_cap) {
+          ppc_hip_adj_record rc;
+          rc.photon = ph.idx, rc.x0 = ph.x0, rc.c0 = ph.c0, rc.lane = gid;
+          P.hit_photons[slot] = rc;
+        }
+      }
     } else {
       if (EXT && P.sum > 0 && fin > 0) {  // birefringence (gpu/pro.cu:757-836)
         float ra, rb;

@@ -54,9 +54,10 @@ def test_adjoint_matches_oracle_photon_by_photon(env):
     rng = np.random.default_rng(7)
     grad_mu = rng.uniform(0.5, 1.5, size=mu.shape)
     ga, gb = sim.backward(launch, grad_mu)
-    assert sim.last_adjoint["records"] > 1000
-    assert sim.last_adjoint["missed"] <= 0.002 * sim.last_adjoint["records"]
-    rec = adjoint.adjoint_records(launch["hits"], launch["photon"], launch["starts"], grad_mu)
+    rec = sim.dev.hit_photons()
+    assert len(rec) == launch["stats"].hit_photons > 1000
+    assert sim.last_adjoint["missed"] == 0  # both runs of every photon agree
+    assert sim.last_adjoint["replays"] == 2 * len(rec)
     rm = det.array("rm", np.uint32)
     T = U.product_tables(det)
     states = [(int(r["x0"]), int(r["c0"]), int(rm[r["lane"]])) for r in rec]
@@ -87,7 +88,7 @@ def test_absorption_scores_sum_to_path_lengths_at_scale(env):
     L = h["t"].astype(np.float64) / env["ocm"][h["z"]]
     want = -np.sum(w[h["n"], h["i"]] * env["lk"][h["z"]] * L)
     assert np.sum(ga / env["dpda"]) == pytest.approx(want, rel=2e-3)
-    assert sim.last_adjoint["missed"] <= 0.002 * sim.last_adjoint["records"]
+    assert sim.last_adjoint["missed"] == 0
 
 
 def _crn_check(env, which, direction, eps, seeds=4, n=4_000_000):
@@ -157,7 +158,7 @@ def test_autograd_poisson_llh(env):
     mu = adjoint.ExpectedHits.apply(a, b, sim)
     llh = adjoint.poisson_llh(mu, k)
     llh.backward()
-    # the same launch by hand: d llh / d mu = k / mu - 1 where mu > 0
+    # the same launch by hand: d llh / d mu = k / mu - 1 where mu > 0 (-1 where the floor clamps)
     sim.dev.set_states(S)
     mu2, launch = sim.forward(sim.par_a, sim.par_b)
     assert np.array_equal(mu.detach().numpy(), mu2)
