@@ -191,7 +191,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   }
   // 16 waves per CU (4 per SIMD: the kernel's register budget allows no more)
   if (k.blocks <= 0) k.blocks = c->cus * (1024 / k.threads);
-  if (k.refill <= 0) k.refill = 16;
+  if (k.refill <= 0) k.refill = 8;
   if (k.chunk > 1024) return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);
   if (k.refill > 64) k.refill = 64;
   if (k.max_bunches == 0) k.max_bunches = 1u << 20;

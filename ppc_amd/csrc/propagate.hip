@@ -482,6 +482,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
 
   while (true) {
     // -------- refill idle lanes --------
+    if (ADJ && done && __any(pend)) done = false;  // a summed photon still has its scoring pass to run
     if (!done) {
       const uint64_t idle = __ballot(!live);
       const int nidle = __popcll(idle);

@@ -91,16 +91,21 @@ def test_absorption_scores_sum_to_path_lengths_at_scale(env):
     assert sim.last_adjoint["missed"] == 0
 
 
-def _crn_check(env, which, direction, eps, seeds=4, n=4_000_000):
-    """LR directional derivative vs CRN central differences of sum_d w_d N_d."""
+def _crn_check(env, which, direction, eps, repeats=24):
+    """LR directional derivative vs CRN central differences of sum_d w_d N_d.
+
+    Every launch runs one photon per lane from chosen lane states, so photon k of
+    the plus, minus and nominal launches starts from the same MWC state (common
+    random numbers photon by photon); the repeats give independent estimates."""
     from ppc_amd import adjoint
 
     det = env["det"]
-    sim = adjoint.HitSimulation(det, point_bunch(n), deterministic=True, max_hits=4 << 20)
+    sim = adjoint.HitSimulation(det, point_bunch(1), deterministic=True)
+    sim.bunches[0].num = sim.num = sim.dev.lanes
     a0, b0 = sim.par_a.astype(np.float64), sim.par_b.astype(np.float64)
     w = np.random.default_rng(11).uniform(0.5, 1.5, size=(1, env["gsize"]))
     lr, fd = [], []
-    for k in range(seeds):
+    for k in range(repeats):
         S = _states(sim.dev.lanes, 100 + k)
         sim.dev.set_states(S)
         mu, launch = sim.forward(a0, b0)
@@ -117,8 +122,9 @@ def _crn_check(env, which, direction, eps, seeds=4, n=4_000_000):
         fd.append((val[0] - val[1]) / (2 * eps))
     sim.close()
     lr, fd = np.array(lr), np.array(fd)
-    err = np.sqrt((lr.var(ddof=1) + fd.var(ddof=1)) / seeds)
-    print("%s: LR %s  FD %s  err %.4g" % (which, lr, fd, err), flush=True)
+    err = np.sqrt((lr.var(ddof=1) + fd.var(ddof=1)) / repeats)
+    print("%s: LR %.5g +- %.3g  FD %.5g +- %.3g" % (which, lr.mean(), lr.std(ddof=1) / np.sqrt(repeats), fd.mean(),
+                                                    fd.std(ddof=1) / np.sqrt(repeats)), flush=True)
     assert abs(lr.mean()) > 5 * err  # the comparison is informative
     assert abs(lr.mean() - fd.mean()) <= 4 * err + 0.02 * abs(fd.mean())
 
@@ -165,5 +171,6 @@ def test_autograd_poisson_llh(env):
     wmu = np.where(mu2 > 0, k / np.maximum(mu2, 1e-9) - 1, -1.0)
     ga, gb = sim.backward(launch, wmu)
     sim.close()
+    assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
     assert np.allclose(a.grad.numpy(), ga, rtol=1e-4, atol=1e-6 * np.abs(ga).max())
     assert np.allclose(b.grad.numpy(), gb, rtol=1e-4, atol=1e-6 * np.abs(gb).max())
