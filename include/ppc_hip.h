@@ -146,7 +146,7 @@ typedef struct {
 /* launch shape and capacities (the reference derives these from the device,
  * gpu/ppc.cu:141-186; 0 selects the library default) */
 typedef struct {
-  int32_t blocks;          /* workgroups; default: 1024/threads per CU (16 waves per CU) */
+  int32_t blocks;          /* workgroups; default: 1792/threads per CU (28 waves per CU, 7 per SIMD) */
   int32_t threads;         /* lanes per workgroup (multiple of 64); default 256 */
   int32_t refill;          /* refill a wave once this many of its 64 lanes are idle; default 8 */
   int32_t deterministic;   /* 1: photon i always runs on lane i % lanes (the reference's DTMN) */
@@ -158,6 +158,7 @@ typedef struct {
   int32_t record_starts;   /* 1: record the start state of every photon that hits (adjoint), max_hits of them */
   uint32_t chunk;          /* photons a wave claims per atomic on the work counter; 0: num/(16 waves), 64..1024 */
   int32_t tilt_lds;        /* 1-d/2-d tilt table: 0 in LDS when it fits in 16 KB, 1 in LDS when it fits, -1 in HBM */
+  int32_t no_clearance;    /* 1: search the cell grid at every step (no skipping of steps far from all strings) */
 } ppc_hip_config;
 
 /* per-launch results (what gpu/ppc.cu:255-283 reads back) */

@@ -52,7 +52,8 @@ class Config(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32), ("refill", ctypes.c_int32),
                 ("deterministic", ctypes.c_int32), ("max_bunches", ctypes.c_uint32), ("max_hits", ctypes.c_uint32),
                 ("rec_stride", ctypes.c_uint32), ("rec_max", ctypes.c_uint32), ("count_events", ctypes.c_int32),
-                ("record_starts", ctypes.c_int32), ("chunk", ctypes.c_uint32), ("tilt_lds", ctypes.c_int32)]
+                ("record_starts", ctypes.c_int32), ("chunk", ctypes.c_uint32), ("tilt_lds", ctypes.c_int32),
+                ("no_clearance", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -246,10 +247,12 @@ class Device:
     """One GPU context of libppc_hip.so."""
 
     def __init__(self, device=0, blocks=0, threads=0, refill=0, deterministic=False, max_bunches=0, max_hits=0,
-                 rec_stride=0, rec_max=0, count_events=False, record_starts=False, chunk=0, tilt_lds=0):
+                 rec_stride=0, rec_max=0, count_events=False, record_starts=False, chunk=0, tilt_lds=0,
+                 no_clearance=False):
         self.lib = hip()
         self.cfg = Config(blocks, threads, refill, 1 if deterministic else 0, max_bunches, max_hits, rec_stride,
-                          rec_max, 1 if count_events else 0, 1 if record_starts else 0, chunk, tilt_lds)
+                          rec_max, 1 if count_events else 0, 1 if record_starts else 0, chunk, tilt_lds,
+                          1 if no_clearance else 0)
         self.ctx = ctypes.c_void_p()
         _check(self.lib.ppc_hip_open(device, ctypes.byref(self.cfg), ctypes.byref(self.ctx)))
         n = ctypes.c_int32()

@@ -18,6 +18,9 @@ constexpr int kLMAX = PPC_HIP_LMAX;
 constexpr int kWNUM = PPC_HIP_WNUM;
 constexpr int kCX = PPC_HIP_CX, kCY = PPC_HIP_CY;
 constexpr int kCTY = PPC_HIP_CTY;
+// Waves per SIMD the propagation kernel is compiled for (the register budget: 72 VGPRs at
+// 7 waves); the default launch fills every SIMD with that many waves
+constexpr int kWavesPerSimd = 7;
 
 // global counters of one launch
 struct Counters {
@@ -43,6 +46,7 @@ struct KParams {
   float cb00, cb01, cb10, cb11;
   int tmod, vthk, lnum, lpts;
   float lmin, lrdz, r0, lnx, lny;
+  float lr[kLMAX];  // 1-d tilt reference distances (also in the LDS image for per-lane indexing)
   float mmin0, mmin1, mstp0, mstp1;
   int mnum0, mnum1, mcut0, mcut1;
   int nsc, gsize;
@@ -61,10 +65,17 @@ struct KParams {
   const ppc_hip_wv* wv;     // [512]
   const float4* dom;        // [gsize] x, y, z, R
   const float* domF;        // [gsize]
+  // clearance map: per square of side 1/cm_rstep over the strings' footprint, a lower
+  // bound of the horizontal distance from any point of the square to the nearest
+  // string cylinder (radius of its line, or of the hole ice if larger)
+  const float* clear_map;   // [cm_ny][cm_nx], or nullptr
+  float cm_x0, cm_y0, cm_rstep;
+  int cm_nx, cm_ny;
 
   // ---- RNG ----
   const uint32_t* rm;  // multiplier per lane
   uint2* rs;           // (x, c) per lane
+  uint4* lane_info;    // per lane: photon index, segment, MWC state (x, c) it started from
 
   // ---- launch ----
   const ppc_hip_bunch* pz;

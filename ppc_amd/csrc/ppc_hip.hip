@@ -93,6 +93,8 @@ struct ppc_hip_ctx {
   int nvar = 1;                   // ice variants in the layer table
   uint16_t* d_bvar = nullptr;     // per-bunch ice variant of a batch launch
   uint16_t* h_bvar = nullptr;
+  float* d_clear = nullptr;       // clearance map of the string cylinders
+  uint4* d_info = nullptr;        // per lane: photon, segment, start state
   ppc_hip_adj_record* d_hphot = nullptr;  // photons that hit (record_starts)
   uint32_t fwd_nphot = 0;         // their number in the last waited-for launch
   bool fwd_waited = false;
@@ -190,7 +192,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
     return fail(PPC_HIP_E_ARG, "threads per workgroup must be a multiple of 64 up to 1024 (got %d)", k.threads);
   }
   // 16 waves per CU (4 per SIMD: the kernel's register budget allows no more)
-  if (k.blocks <= 0) k.blocks = c->cus * (1024 / k.threads);
+  if (k.blocks <= 0) k.blocks = c->cus * std::max(1, ppc::kWavesPerSimd * 4 * 64 / k.threads);
   if (k.refill <= 0) k.refill = 8;
   if (k.chunk > 1024) return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);
   if (k.refill > 64) k.refill = 64;
@@ -224,6 +226,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(dalloc(&c->d_bvar, k.max_bunches));
   chk(hipHostMalloc((void**)&c->h_bvar, sizeof(uint16_t) * k.max_bunches, hipHostMallocDefault));
   if (k.record_starts) chk(dalloc(&c->d_hphot, k.max_hits));
+  chk(dalloc(&c->d_info, c->lanes));
   chk(dalloc(&c->d_ctr_adj, 1));
   chk(hipHostMalloc((void**)&c->h_ctr_adj, sizeof(ppc::Counters), hipHostMallocDefault));
   if (e != hipSuccess) {
@@ -241,7 +244,7 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj,
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj, (void*)c->d_clear, (void*)c->d_info,
                   (void*)c->d_adj_rec, (void*)c->d_adj_w, (void*)c->d_adj_dp, (void*)c->d_adj_grad})
     if (p) (void)hipFree(p);
   drop_graph(c);
@@ -322,6 +325,7 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.cb00 = t->cb[0], P.cb01 = t->cb[1], P.cb10 = t->cb[2], P.cb11 = t->cb[3];
   P.tmod = t->tmod, P.vthk = t->vthk, P.lnum = t->lnum, P.lpts = t->lpts;
   P.lmin = t->lmin, P.lrdz = t->lrdz, P.r0 = t->r0, P.lnx = t->lnx, P.lny = t->lny;
+  for (int i = 0; i < PPC_HIP_LMAX; i++) P.lr[i] = t->lr[i];
   P.mmin0 = t->mmin[0], P.mmin1 = t->mmin[1], P.mstp0 = t->mstp[0], P.mstp1 = t->mstp[1];
   P.mnum0 = t->mnum[0], P.mnum1 = t->mnum[1], P.mcut0 = t->mcut[0], P.mcut1 = t->mcut[1];
   P.nsc = t->nsc, P.gsize = t->gsize;
@@ -406,6 +410,41 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     HIPCHK(hipMemcpy(c->d_domF, domF.data(), sizeof(float) * domF.size(), hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(c->d_rm, t->rm, sizeof(uint32_t) * c->lanes, hipMemcpyHostToDevice));
+  {  // clearance map of the string cylinders: squares of 4 m over the strings' footprint + 100 m
+    std::vector<float> cm;
+    int nx = 0, ny = 0;
+    float x0 = 0, y0 = 0;
+    const float step = 4.f;
+    if (t->nsc > 0 && !c->cfg.no_clearance) {
+      float xl = t->sc[0].x, xh = xl, yl = t->sc[0].y, yh = yl;
+      for (int i = 1; i < t->nsc; i++) {
+        xl = std::min(xl, t->sc[i].x), xh = std::max(xh, t->sc[i].x);
+        yl = std::min(yl, t->sc[i].y), yh = std::max(yh, t->sc[i].y);
+      }
+      x0 = xl - 100.f, y0 = yl - 100.f;
+      nx = (int)std::ceil((xh + 100.f - x0) / step), ny = (int)std::ceil((yh + 100.f - y0) / step);
+      cm.resize((size_t)nx * ny);
+      // a point of the square is at most half a diagonal from its centre; 0.05 m covers rounding
+      const double slack = step * 0.70710678 + 0.05;
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const double cx = x0 + (i + 0.5) * step, cy = y0 + (j + 0.5) * step;
+          double best = 1e30;
+          for (int k = 0; k < t->nsc; k++) {
+            const ppc_hip_line& l = t->sc[k];
+            best = std::min(best, std::hypot(cx - l.x, cy - l.y) - std::max<double>(l.r, t->hr));
+          }
+          cm[(size_t)j * nx + i] = (float)(best - slack);
+        }
+    }
+    if (c->d_clear) (void)hipFree(c->d_clear), c->d_clear = nullptr;
+    if (!cm.empty()) {
+      HIPCHK(dalloc(&c->d_clear, cm.size()));
+      HIPCHK(hipMemcpy(c->d_clear, cm.data(), sizeof(float) * cm.size(), hipMemcpyHostToDevice));
+    }
+    P.clear_map = c->d_clear;
+    P.cm_x0 = x0, P.cm_y0 = y0, P.cm_rstep = 1.f / step, P.cm_nx = nx, P.cm_ny = ny;
+  }
 
   P.lp = c->d_lp;
   P.wv = c->d_wv;
@@ -413,6 +452,7 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.domF = c->d_domF;
   P.rm = c->d_rm;
   P.rs = c->d_rs;
+  P.lane_info = c->d_info;
   P.pz = c->d_pz;
   P.pstart = c->d_pstart;
   P.hits = c->d_hits;

@@ -172,9 +172,10 @@ template <int TILT>
 __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const float4& r, float& dz) {
   if constexpr (TILT == 1) {
     float nr = P.lnx * r.x + P.lny * r.y - P.r0;
-    int j = 1;
-    for (; j < kLMAX; j++)
-      if (nr < T.lr[j] || j == P.lnum - 1) break;
+    // first j >= 1 with nr < lr[j], else lnum - 1: a loop of uniform trip count over
+    // the reference points (scalar loads of lr), so lanes do not diverge
+    int j = P.lnum - 1;
+    for (int q = P.lnum - 2; q >= 1; q--) j = nr < P.lr[q] ? q : j;
     float z = (r.z - P.lmin) * P.lrdz;
     int i = j - 1, k = min(max(rd(z), 0), P.lpts);
     float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
@@ -276,15 +277,14 @@ struct Photon {
   float4 r, n;  // n.w = 1/group velocity
   float TOT, SCA;
   int old, ofla;
-  uint32_t seg, wbin, idx;
-  uint32_t x0, c0;  // MWC state before emission (replay records)
-  uint32_t nh;      // hits of this photon so far
+  uint32_t nhw;     // hits of this photon so far (low 16 bits), wavelength bin << 16
+  float clear;      // lower bound of the horizontal distance to every string cylinder (0: unknown)
   Wave w;
 };
 
 // Emit photon i: pick its bunch, a wavelength bin, and the source-specific
 // start and direction (gpu/pro.cu:334-477).  Returns the starting DOM (fla).
-__device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
+__device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph, uint32_t& seg) {
   // bunch: last b with pstart[b] <= i
   uint32_t lo = 0, hi = P.nb;
   while (hi - lo > 1) {
@@ -413,8 +413,8 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
     }
   }
   ph.r = r, ph.n = n;
-  ph.seg = p.q;
-  ph.wbin = wv.wvl;
+  ph.nhw = (uint32_t)wv.wvl << 16;
+  seg = p.q;
   ph.w.la = wv.la, ph.w.lk = wv.lk, ph.w.abl = wv.abl;
   ph.w.lay = P.bvar ? (int)P.bvar[lo] * P.size : 0;
   ph.ofla = ofla;
@@ -426,13 +426,17 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph) {
 // The parameter block is read through a pointer into device memory: uniform
 // loads become scalar loads the compiler can repeat, where a by-value argument
 // of this size would be held in SGPRs and spilled into VGPR lanes.
+// Occupancy: the kernel is VALU-issue bound with divergent branches, so more
+// waves per SIMD keep the VALU busier; at 7 waves the register budget is 72 VGPRs
+// (a few spills in the cold emission code), measured faster than 4, 5, 6 and 8.
 // ADJ: replay the photons of adjoint records (each from its recorded MWC state)
 // and accumulate d(LLH)/d(per-layer absorption, scattering) instead of hits:
 // along every path segment the score of the path density, -nr*d(abs)/da*l for
 // absorption and d(sca)/db*(1/sca at a scattering - l) for scattering, weighted
 // by the LLH weight of the hits the photon still makes after the segment.
 template <int TILT, bool EXT, bool COUNT, bool ADJ>
-__global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void propagate(
+    const KParams* __restrict__ Pp) {
   const KParams& P = *Pp;
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
@@ -451,6 +455,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
   T.ls = reinterpret_cast<const uint16_t*>(base + P.off_ls);
   T.sc = reinterpret_cast<const float4*>(base + P.off_sc);
   T.lp = P.lp_in_lds ? reinterpret_cast<const float2*>(base + P.off_lp) : P.lp;
+
   T.lp_stride = P.lp_stride;
 
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,8 +475,13 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
   // one atomic claims a chunk, so the global counter sees one request per chunk
   uint32_t res_next = 0, res_end = 0;
   const uint32_t chunk = P.chunk;
+  // steps, photons and killed photons of the wave (uniform: scalar registers), non-finite
+  // photons of the lane (rare), event counts of the lane (COUNT builds only)
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
-  uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;  // event counts (COUNT builds only)
+  uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;
+  // per lane in memory: the photon's index, its segment and the MWC state it started from
+  // (read back only for hits, records and the adjoint's second pass)
+  uint4* const info = P.lane_info + gid;
   // ADJ: each record's photon runs twice from its start state: pass 0 sums the LLH
   // weights of its hits (Gsum), pass 1 scores the path with G, the weight of the hits
   // still to come
@@ -490,8 +500,9 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         uint32_t i = 0xffffffffu;
         if (ADJ) {  // one record after the other per lane
           if (!live && pend) {  // the scoring pass of the photon just summed
-            s.x = ph.x0, s.c = ph.c0;
-            i = ph.idx;
+            const uint4 in = *info;
+            s.x = in.z, s.c = in.w;
+            i = in.x;
             G = Gsum, pass1 = true, pend = false;  // nw carries over from pass 0
           } else if (!live) {
             const uint32_t k = next_static;
@@ -530,36 +541,38 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
             }
           }
         }
-        if (!live && i < P.num) {
-          ph.x0 = s.x, ph.c0 = s.c;
+        const bool start = !live && i < P.num;
+        n_photons += __popcll(__ballot(start));
+        if (start) {
+          const uint32_t x0 = s.x, c0 = s.c;
           if (!ADJ && P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
             ppc_hip_record rc;
             rc.photon = i, rc.x = s.x, rc.c = s.c, rc.a = s.a;
             P.rec[i / P.rec_stride] = rc;
           }
-          ph.idx = i;
-          ph.nh = 0;
-          int fla = emit(P, i, s, ph);
-          n_photons++;
+          ph.clear = 0.f;
+          uint32_t seg;
+          int fla = emit(P, i, s, ph, seg);
+          *info = make_uint4(i, seg, x0, c0);
           ph.TOT = -flog(s());
           ph.SCA = 0;
           ph.old = fla;
           if (ph.ofla < -1) {  // killed at emission: the reference still draws SCA
             s();
             ph.TOT = 0;
-            n_killed++;
           } else
             live = true;
         }
+        n_killed += __popcll(__ballot(start && !live));
       }
     }
     if (!__any(live)) {
       if (done) break;
       continue;
     }
+    n_steps += __popcll(__ballot(live));
     if (live) {
     // -------- one transport step (gpu/pro.cu:486-889) --------
-    n_steps++;
     float4 r = ph.r, n = ph.n;
     float TOT = ph.TOT, SCA = ph.SCA;
     int old = ph.old;
@@ -629,7 +642,30 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
     int om = -1;
     float del = sca, sqd = 0;
     float hi = sca, hf = 0;
-    {  // DOM and hole-ice search in the cells overlapping the step
+    const float np = 1 - n.z * n.z;
+    // DOM and hole-ice search in the cells overlapping the step, unless the step cannot
+    // reach any string cylinder: its horizontal length is below the clearance of its
+    // start, a lower bound of the distance to every cylinder that the photon carries from
+    // step to step (less the horizontal length of each step) and renews from the clearance
+    // map when it runs short.  The cylinder test of every string would fail on such a
+    // step, so the skip changes no result.
+    bool search = true;
+    if (P.clear_map) {
+      const float reach = sca * fsqrt(np);
+      if (ph.clear <= reach) {
+        ph.clear = 0.f;
+        const float gx = (r.x - P.cm_x0) * P.cm_rstep, gy = (r.y - P.cm_y0) * P.cm_rstep;
+        if (gx >= 0.f && gy >= 0.f && gx < (float)P.cm_nx && gy < (float)P.cm_ny)
+          ph.clear = P.clear_map[(int)gy * P.cm_nx + (int)gx];
+      }
+      if (ph.clear > reach) {
+        search = false;
+        // a curved path (birefringence) ends off the straight line: renew every step
+        ph.clear = (EXT && P.sum > 0) ? 0.f : ph.clear - reach;
+      } else
+        ph.clear = 0.f;
+    }
+    if (search) {
       float rfx = r.x + sca * n.x, rfy = r.y + sca * n.y;
       float pix = P.cb00 * r.x + P.cb10 * r.y, piy = P.cb01 * r.x + P.cb11 * r.y;
       float pfx = P.cb00 * rfx + P.cb10 * rfy, pfy = P.cb01 * rfx + P.cb11 * rfy;
@@ -639,7 +675,6 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       int xhx = max(min(rn((hix - P.cl0) * P.crst0), P.cn0 - 1), -1);
       int xly = min(max(rn((loy - P.cl1) * P.crst1), 0), P.cn1);
       int xhy = max(min(rn((hiy - P.cl1) * P.crst1), P.cn1 - 1), -1);
-      const float np = 1 - n.z * n.z;
       for (int ci = xlx; ci <= xhx; ci++)
         for (int cj = xly; cj <= xhy; cj++)
           for (uint32_t k = T.is[ci * kCY + cj]; k != 0x8000u;) {
@@ -775,8 +810,9 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       if (om == ofla)
         TOT = 0;
       else {
-        if (ph.seg < P.adj_nseg) {
-          const float wh = P.adj_weight[(size_t)ph.seg * P.gsize + om];
+        const uint32_t seg = info->y;
+        if (seg < P.adj_nseg) {
+          const float wh = P.adj_weight[(size_t)seg * P.gsize + om];
           if (!pass1)
             Gsum += wh, nw++;
           else if (nw > 0)
@@ -792,7 +828,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         TOT = 0;
       else {
         ppc_hip_hit h;
-        h.i = om, h.t = r.w, h.n = ph.seg, h.z = ph.wbin;
+        const uint4 in = *info;
+        h.i = om, h.t = r.w, h.n = in.y, h.z = ph.nhw >> 16;
         h.pth = acosf(n.z);
         h.pph = atan2f(n.y, n.x);
         sqd *= P.xR - 1;
@@ -814,9 +851,9 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         const uint32_t slot = first + __popcll(hm & ((1ull << lane) - 1));
         if (slot < P.hnum) {
           P.hits[slot] = h;
-          P.hit_photon[slot] = ph.idx;
+          P.hit_photon[slot] = in.x;
         }
-        ph.nh++;
+        ph.nhw++;
         if (P.xR == 1)
           TOT = 0;
         else
@@ -830,7 +867,7 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
       if (ADJ) {
         if (!pass1 && nw > 0) pend = true;
         if (pass1 && nw != 0) n_nonfinite++;  // the scoring pass lost a hit of the summing pass
-      } else if (P.hit_photons && ph.nh) {  // start state of a photon that hit, for the adjoint
+      } else if (P.hit_photons && (ph.nhw & 0xffffu)) {  // start state of a photon that hit, for the adjoint
         const uint64_t em = __ballot(true);
         const int leader = __ffsll((unsigned long long)em) - 1;
         uint32_t first = 0;
@@ -839,7 +876,8 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
         const uint32_t slot = first + __popcll(em & ((1ull << lane) - 1));
         if (slot < P.hit_photons_cap) {
           ppc_hip_adj_record rc;
-          rc.photon = ph.idx, rc.x0 = ph.x0, rc.c0 = ph.c0, rc.lane = gid;
+          const uint4 in = *info;
+          rc.photon = in.x, rc.x0 = in.z, rc.c0 = in.w, rc.lane = gid;
           P.hit_photons[slot] = rc;
         }
       }
@@ -967,10 +1005,11 @@ __global__ __launch_bounds__(1024) void propagate(const KParams* __restrict__ Pp
   } else
     P.rs[gid] = make_uint2(s.x, s.c);
 
-  // per-launch counters: one atomic per wave
+  // per-launch counters: one atomic per wave; steps, photons and killed photons are
+  // already wave totals, the others are summed over the lanes
   constexpr int NC = COUNT ? 8 : 4;
   unsigned long long v[8] = {n_steps, n_photons, n_killed, n_nonfinite, n_cross, n_str, n_dom, n_scat};
-  for (int q = 0; q < NC; q++) {
+  for (int q = 3; q < NC; q++) {
     unsigned long long t = v[q];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
     v[q] = t;

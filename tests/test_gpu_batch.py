@@ -111,3 +111,21 @@ def test_graph_replays_equal_eager_launches(det):
         dev.run([other])  # eager work between replays
         print("replay %d: %d hits equal" % (k, len(hg)), flush=True)
     dev.close()
+
+
+def test_clearance_skip_changes_no_hit(det):
+    """Steps that cannot reach any string skip the cell-grid search; deterministic
+    launches with and without that skip give identical hits."""
+    from ppc_amd import api
+
+    out = []
+    for off in (False, True):
+        dev = api.Device(0, deterministic=True, no_clearance=off)
+        dev.upload(det)
+        dev.set_states(_states(dev.lanes, 3))
+        print("clearance skip %s: cascade of %d photons" % ("off" if off else "on", 2 * dev.lanes), flush=True)
+        dev.run([_cascade(det, 2 * dev.lanes)])
+        out.append(_sorted_hits(dev))
+        dev.close()
+    assert len(out[0][0]) > 100
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
