@@ -41,6 +41,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (AMD published)
 
+# HBM bytes per photon of the propagation kernel on this workload, from rocprofv3 PMC runs
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy of known size gives):
+# profiles/propagate_c2_cascade_pmc.md
+HBM_BYTES_PER_PHOTON = 37.2
+
 # Algorithmic FP32 operations of the reference algorithm per event, counted from
 # gpu/pro.cu (add/mul = 1, FMA = 2; divide, sqrt, rsqrt, log, exp, sin, cos = 1).
 FLOPS = {
@@ -105,7 +110,7 @@ def gpu_run(args):
         "flop_per_photon": fl_photon,
         "events_per_photon": per,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_FP32, 4), "traffic": round(HBM_BYTES_PER_PHOTON * photons)},
         "cascade": cas,
     }
 
