@@ -154,7 +154,9 @@ def adjoint_run(events=12, photons=1e8, seed=6):
 
 
 def batch_run(steps, seed=5):
-    """C4: 512 mixed sources in 8 ice variants, captured graph, autotuned launch shape"""
+    """C4: 512 mixed sources in 8 ice variants, captured graphs, autotuned launch shape.
+    Photon indices are 32-bit, so the 8e9 photons of the batch are split into consecutive
+    groups of at most 3e9, each captured by its own context; a step replays all groups."""
     from ppc_amd import api, configs, tune
 
     det = api.Detector(configs.ensure("c1"), {"WFLA": "405", "FWID": "10", "DTMN": None})
@@ -162,28 +164,43 @@ def batch_run(steps, seed=5):
     num = sum(b.num for b in bunches)
     best, table = tune.tune(det, bunches, variant, layers, photons=2e8, repeats=1)
     cus = tune.device_cus()
-    dev = api.Device(0, max_bunches=len(bunches), max_hits=1 << 26, **tune.device_kwargs(best, cus))
-    dev.upload(det)
-    dev.set_layers(layers)
-    dev.graph_capture(bunches, variant, num)
-    dev.graph_launch()
-    dev.wait()
-    ms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    groups, cur, tot = [], [], 0
+    for k, b in enumerate(bunches):
+        if cur and tot + b.num > 3e9:
+            groups.append(cur)
+            cur, tot = [], 0
+        cur.append(k)
+        tot += b.num
+    groups.append(cur)
+    devs = []
+    for g in groups:
+        dev = api.Device(0, max_bunches=len(g), max_hits=1 << 25, **tune.device_kwargs(best, cus))
+        dev.upload(det)
+        dev.set_layers(layers)
+        dev.graph_capture([bunches[k] for k in g], variant[g], sum(bunches[k].num for k in g))
+        devs.append(dev)
+    for dev in devs:
         dev.graph_launch()
-        st = dev.wait()
-        ms.append(st.device_ms)
-        assert st.photons == num
+    for dev in devs:
+        dev.wait()
+    t0 = time.perf_counter()
+    photons = 0
+    for _ in range(steps):
+        for dev in devs:
+            dev.graph_launch()
+        for dev in devs:
+            photons += dev.wait().photons
     wall = time.perf_counter() - t0
-    dev.close()
+    for dev in devs:
+        dev.close()
+    assert photons == num * steps
     return {"workload": "512 sources (flashers, cascades, 10 m muon segments), photons log-uniform in [1e5, 1e8], "
-                        "8 ice variants of the 7-string ice", "photons_per_step": num,
-            "photons_per_s": round(num * steps / wall, 1), "graph_ms": round(sum(ms) / len(ms), 2),
+                        "8 ice variants of the 7-string ice", "photons_per_step": num, "graphs": len(devs),
+            "photons_per_s": round(num * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 2),
             "tuned_shape": best, "candidates": len(table),
-            "untuned_default_ms": next((round(t, 2) for sh, t in table
-                                        if sh == {"threads": 256, "waves_per_cu": 28, "refill": 8, "chunk": 0,
-                                                  "tilt_lds": 0}), None),
+            "untuned_default_ms_at_2e8": next((round(t, 2) for sh, t in table
+                                               if sh == {"threads": 256, "waves_per_cu": 28, "refill": 8, "chunk": 0,
+                                                         "tilt_lds": 0}), None),
             "tuned_ms_at_2e8": round(min(t for _, t in table), 2)}
 
 

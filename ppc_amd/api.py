@@ -137,6 +137,16 @@ def xppc():
     return _xppc
 
 
+MAX_PHOTONS = 2 ** 32 - 2 ** 24  # per launch: photon indices are 32-bit, the work counter overshoots by a chunk per wave
+
+
+def _num(num):
+    num = int(num)
+    if num < 0 or num > MAX_PHOTONS:
+        raise PPCError("%d photons in one launch: split the bunches over launches of at most %d" % (num, MAX_PHOTONS))
+    return num
+
+
 def _check(rc):
     if rc != 0:
         raise PPCError("ppc_hip error %d: %s" % (rc, hip().ppc_hip_last_error().decode()))
@@ -295,7 +305,7 @@ class Device:
         arr = (Bunch * len(bunches))(*bunches)
         if num is None:
             num = sum(b.num for b in bunches)
-        _check(self.lib.ppc_hip_launch(self.ctx, arr, len(bunches), num))
+        _check(self.lib.ppc_hip_launch(self.ctx, arr, len(bunches), _num(num)))
 
     def wait(self):
         st = Stats()
@@ -351,7 +361,7 @@ class Device:
         if num is None:
             num = sum(b.num for b in arr)
         _check(self.lib.ppc_hip_launch_batch(self.ctx, arr, v.ctypes.data_as(ctypes.c_void_p) if v is not None
-                                             else None, len(arr), num))
+                                             else None, len(arr), _num(num)))
 
     def graph_capture(self, bunches, variant=None, num=None):
         arr, v = self._bunches(bunches, variant)
@@ -359,7 +369,7 @@ class Device:
             num = sum(b.num for b in arr)
         self._graph_keep = (arr, v)
         _check(self.lib.ppc_hip_graph_capture(self.ctx, arr, v.ctypes.data_as(ctypes.c_void_p) if v is not None
-                                              else None, len(arr), num))
+                                              else None, len(arr), _num(num)))
 
     def graph_launch(self):
         _check(self.lib.ppc_hip_graph_launch(self.ctx))
