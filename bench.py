@@ -20,6 +20,9 @@ The JSON line also carries
                 each, 7-string ice, 100 layers: 200 parameters): forward and
                 adjoint time, their ratio, and the cost of the same gradient by
                 central differences (2 x 200 forward passes);
+  end_to_end    the same cascade through the library-mode pipeline llh drives
+                (xppc::sett/addp/eout into hitz): bunch upload, launches, hit
+                download and the host's hit post-processing, by wall clock;
   batch         (--batch) 512 sources of mixed types in 8 ice variants on the
                 7-string ice, one captured-graph replay per step, at the launch
                 shape the autotuner picks.
@@ -113,6 +116,37 @@ def gpu_run(args):
                      "frac": round(achieved / PEAK_FP32, 4), "traffic": round(HBM_BYTES_PER_PHOTON * photons)},
         "cascade": cas,
     }
+
+
+def end_to_end_run():
+    """the library-mode pipeline llh drives (xppc: sett + addp of the cascade, eout, hitz):
+    bunch upload, launches, hit download and the reference's host post-processing into hitz"""
+    from ppc_amd import api, configs
+
+    cas = configs.c2_cascade()
+    det = api.Detector(configs.ensure("c2"), {"WFLA": None, "FWID": None, "DTMN": None})
+    n = configs.f2k_direction(cas["th"], cas["ph"])
+    det.start(0)
+    lib = det.lib
+
+    def event():
+        lib.xppc_c_sett(n[0], n[1], n[2], 1, 0, 0, 0.0)
+        lib.xppc_c_addp(cas["x"], cas["y"], cas["z"], cas["t"], cas["E"], 0.0, 11, 1.0)
+        lib.xppc_c_eout()
+        k = int(lib.xppc_c_hitz(None, None, None, None, None, 0))
+        lib.xppc_c_efin()
+        return k
+
+    event()  # warm-up
+    p0, h0, ms0 = det.totals()
+    t0 = time.perf_counter()
+    acc = event()
+    wall = time.perf_counter() - t0
+    p1, h1, ms1 = det.totals()
+    det.stop()
+    return {"workload": "one 100 TeV cascade through xppc::addp/eout into hitz (86-string ice)",
+            "photons": p1 - p0, "raw_hits": h1 - h0, "accepted_hits": acc, "wall_s": round(wall, 4),
+            "kernel_ms": round(ms1 - ms0, 2), "photons_per_s": round((p1 - p0) / wall, 1)}
 
 
 def adjoint_run(events=12, photons=1e8, seed=6):
@@ -295,6 +329,7 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
     g = gpu_run(args)
+    e2e = None if args.no_adjoint else end_to_end_run()
     adj = None if args.no_adjoint else adjoint_run()
     bat = batch_run(args.steps) if args.batch else None
     cpu = None
@@ -325,6 +360,8 @@ def main():
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
     }
+    if e2e:
+        out["end_to_end"] = e2e
     if adj:
         out["adjoint"] = adj
     if bat:

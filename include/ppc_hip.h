@@ -262,6 +262,57 @@ int ppc_hip_adjoint(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb,
                     const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
                     const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* stats);
 
+/* ---- hit post-processing on the device ---- */
+/* sensor type: angular sensitivity and PMT layout (gpu/ini.cxx:122-281) */
+typedef struct {
+  int32_t def;        /* 1: one PMT with an angular-sensitivity polynomial; 0: PMTs along pmt_dirs */
+  int32_t ns;         /* polynomial coefficients in s (def) */
+  int32_t npmt, pmt0; /* PMTs of the type: pmt_dirs[pmt0 .. pmt0+npmt) (not def) */
+  float mas;          /* def: maximum of the polynomial (<= 0: cut pos.tilt > s[0]) */
+  float beta, fx, Rr, Rz, cable; /* not def: PMT acceptance curve, its normalisation, size, cable azimuth */
+  float s[16];
+} ppc_hip_sensor;
+
+/* per DOM: what the acceptance of its hits needs (gpu/ini.cxx:86-98, gpu/f2k.cxx:266-324) */
+typedef struct {
+  int32_t sensor;   /* index into sensors, -1: the DOM records nothing (no high voltage or no acceptance) */
+  int32_t acc;      /* row of rat */
+  float rde;        /* the DOM's relative efficiency over the maximum of its acceptance class */
+  float azi;        /* cable azimuth [deg], < 0: no cable */
+  float tilt[3];    /* DOM axis */
+  float R, F;       /* radius, axis ratio */
+  float pad;
+} ppc_hip_dom_accept;
+
+typedef struct {
+  int32_t nsensor;
+  const ppc_hip_sensor* sensors;
+  int32_t npmt;
+  const float* pmt_dirs;          /* [npmt][3] */
+  int32_t nacc;
+  const float* rat;               /* [nacc][PPC_HIP_WNUM] acceptance of a wavelength bin */
+  int32_t gsize;
+  const ppc_hip_dom_accept* doms; /* [gsize] */
+  const float* wvs;               /* [PPC_HIP_WNUM][2] low and high wavelength of a bin [nm] */
+} ppc_hip_accept_tables;
+
+/* the fate of one raw hit: accepted by PMT pmt (>= 0) or rejected (-1); for an accepted hit
+ * the photon's wavelength [nm], the impact point relative to the DOM centre [m] and the
+ * photon direction (the reference's pout, gpu/f2k.cxx:340-352) */
+typedef struct {
+  int32_t pmt;
+  float wv;
+  float r[3], n[3];
+} ppc_hip_accepted;
+
+/* copy the acceptance tables to the device (no counterpart: the reference applies them on
+ * the host in print(), gpu/f2k.cxx:247-363; wavelength-shifting tube sensors are refused) */
+int ppc_hip_upload_accept(ppc_hip_ctx* ctx, const ppc_hip_accept_tables* tables);
+/* decide acceptance, PMT and wavelength of the min(stored, cap) hits of the last launch on the
+ * device, one record per hit in the order of ppc_hip_hits, with counter-based random numbers
+ * of seed (statistically the reference's host draws, not the same numbers); synchronous */
+int ppc_hip_accept(ppc_hip_ctx* ctx, uint64_t seed, ppc_hip_accepted* out, uint32_t cap, uint32_t* n);
+
 /* message of the last failing call (thread local) */
 const char* ppc_hip_last_error(void);
 

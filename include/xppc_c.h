@@ -76,6 +76,11 @@ int xppc_c_eout(void);
 /* number of photon records in hitz; with non-NULL arrays, fills up to cap of them */
 uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* wv, uint64_t cap);
 int xppc_c_efin(void);
+/* the acceptance tables of print() in the layout the device post-processing takes
+ * (ppc_hip_upload_accept); valid until the next call */
+const ppc_hip_accept_tables* xppc_c_accept_tables(void);
+/* totals of the launches since start: photons propagated, raw hits, device time [ms] */
+int xppc_c_totals(uint64_t* photons, uint64_t* hits, double* device_ms);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,7 @@ class Hit(ctypes.Structure):
 HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"), ("pph", "<f4"),
                       ("dth", "<f4"), ("dph", "<f4")])
 RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<u4")])
+ACCEPTED_DTYPE = np.dtype([("pmt", "<i4"), ("wv", "<f4"), ("r", "<f4", (3,)), ("n", "<f4", (3,))])
 ADJ_RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x0", "<u4"), ("c0", "<u4"), ("lane", "<u4")])
 LAYER_DTYPE = np.dtype([("s0", "<f4"), ("s1", "<f4"), ("p", "<f4"), ("t", "<f4")])
 
@@ -101,6 +102,8 @@ def hip():
         lib.ppc_hip_graph_capture.argtypes = [c_p, ctypes.POINTER(Bunch), c_p, ctypes.c_uint32, ctypes.c_uint32]
         lib.ppc_hip_graph_launch.argtypes = [c_p]
         lib.ppc_hip_hit_photons.argtypes = [c_p, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.ppc_hip_upload_accept.argtypes = [c_p, c_p]
+        lib.ppc_hip_accept.argtypes = [c_p, ctypes.c_uint64, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         lib.ppc_hip_adjoint.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32, c_p,
                                         ctypes.c_uint32, c_p, ctypes.c_uint32, c_p, c_p, c_p, ctypes.POINTER(Stats)]
         _hip = lib
@@ -133,6 +136,9 @@ def xppc():
         lib.xppc_c_addp.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int32, ctypes.c_float]
         lib.xppc_c_hitz.restype = ctypes.c_uint64
         lib.xppc_c_hitz.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_uint64]
+        lib.xppc_c_accept_tables.restype = ctypes.c_void_p
+        lib.xppc_c_totals.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_double)]
         _xppc = lib
     return _xppc
 
@@ -240,6 +246,12 @@ class Detector:
         lay["p"] = lay["p"] + q["dpda"] * (a - q["par_a"])
         lay["s1"] = lay["s1"] + q["ds1db"] * (b - q["par_b"])
         return lay
+
+    def totals(self):
+        """photons, raw hits and device time [ms] of the library-mode launches since start"""
+        p, h, ms = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        self.lib.xppc_c_totals(ctypes.byref(p), ctypes.byref(h), ctypes.byref(ms))
+        return p.value, h.value, ms.value
 
     def zshift(self, x, y, z):
         dz = ctypes.c_float()
@@ -373,6 +385,21 @@ class Device:
 
     def graph_launch(self):
         _check(self.lib.ppc_hip_graph_launch(self.ctx))
+
+    # ---- hit post-processing ----
+    def upload_accept(self, det):
+        """the acceptance tables of the detector's print() (xppc_c_accept_tables)"""
+        _check(self.lib.ppc_hip_upload_accept(self.ctx, det.lib.xppc_c_accept_tables()))
+
+    def accept(self, seed):
+        """device acceptance of the stored hits of the last launch: ACCEPTED_DTYPE per hit"""
+        st = Stats()
+        _check(self.lib.ppc_hip_wait(self.ctx, ctypes.byref(st)))
+        m = int(st.stored)
+        a = np.zeros(m, dtype=ACCEPTED_DTYPE)
+        n = ctypes.c_uint32()
+        _check(self.lib.ppc_hip_accept(self.ctx, seed, a.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
+        return a[:n.value]
 
     # ---- adjoint ----
     def hit_photons(self):

@@ -102,6 +102,16 @@ struct KParams {
   uint32_t lanes;
 };
 
+// tables of the hit post-processing kernel (accept.hip)
+struct AcceptParams {
+  const ppc_hip_sensor* sensors;
+  const float* pmt_dirs;
+  const float* rat;
+  const ppc_hip_dom_accept* doms;
+  const float* wvs;
+  int32_t gsize;
+};
+
 }  // namespace ppc
 
 #endif

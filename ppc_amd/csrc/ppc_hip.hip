@@ -20,6 +20,8 @@ namespace ppc {
 hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int threads, size_t lds_bytes,
                             hipStream_t stream, bool count, bool adjoint);
 hipError_t launch_reset(Counters* ctr, hipStream_t stream);
+hipError_t launch_accept(const AcceptParams& A, const ppc_hip_hit* hits, const uint32_t* photon, uint32_t n,
+                         uint64_t seed, ppc_hip_accepted* out, hipStream_t stream);
 }
 
 using ppc::KParams;
@@ -95,6 +97,14 @@ struct ppc_hip_ctx {
   uint16_t* h_bvar = nullptr;
   float* d_clear = nullptr;       // clearance map of the string cylinders
   uint4* d_info = nullptr;        // per lane: photon, segment, start state
+  // hit post-processing tables and output (ppc_hip_upload_accept / ppc_hip_accept)
+  ppc_hip_sensor* d_sensors = nullptr;
+  float* d_pmt_dirs = nullptr;
+  float* d_rat = nullptr;
+  ppc_hip_dom_accept* d_adoms = nullptr;
+  float* d_awvs = nullptr;
+  int32_t accept_gsize = -1;
+  ppc_hip_accepted* d_accepted = nullptr;
   ppc_hip_adj_record* d_hphot = nullptr;  // photons that hit (record_starts)
   uint32_t fwd_nphot = 0;         // their number in the last waited-for launch
   bool fwd_waited = false;
@@ -244,7 +254,8 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj, (void*)c->d_clear, (void*)c->d_info,
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj, (void*)c->d_clear, (void*)c->d_info, (void*)c->d_sensors,
+                  (void*)c->d_pmt_dirs, (void*)c->d_rat, (void*)c->d_adoms, (void*)c->d_awvs, (void*)c->d_accepted,
                   (void*)c->d_adj_rec, (void*)c->d_adj_w, (void*)c->d_adj_dp, (void*)c->d_adj_grad})
     if (p) (void)hipFree(p);
   drop_graph(c);
@@ -633,6 +644,63 @@ int ppc_hip_graph_launch(ppc_hip_ctx* c) {
   c->launched = true;
   c->fwd_waited = false;
   c->last_num = c->graph_num;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_upload_accept(ppc_hip_ctx* c, const ppc_hip_accept_tables* t) {
+  if (!c || !t) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (t->nsensor < 0 || t->npmt < 0 || t->nacc < 0 || t->gsize < 0 || (t->nsensor && !t->sensors) ||
+      (t->npmt && !t->pmt_dirs) || (t->nacc && !t->rat) || (t->gsize && !t->doms) || !t->wvs)
+    return fail(PPC_HIP_E_ARG, "bad acceptance tables");
+  for (int k = 0; k < t->nsensor; k++) {
+    const ppc_hip_sensor& S = t->sensors[k];
+    if (!S.def && S.beta <= -2) return fail(PPC_HIP_E_ARG, "sensor %d: wavelength-shifting tubes are not supported", k);
+    if (S.def && (S.ns < 1 || S.ns > 16)) return fail(PPC_HIP_E_ARG, "sensor %d: %d coefficients", k, S.ns);
+    if (!S.def && (S.npmt < 0 || S.pmt0 < 0 || S.pmt0 + S.npmt > t->npmt))
+      return fail(PPC_HIP_E_ARG, "sensor %d: PMTs %d..%d of %d", k, S.pmt0, S.pmt0 + S.npmt, t->npmt);
+  }
+  for (int i = 0; i < t->gsize; i++) {
+    const ppc_hip_dom_accept& d = t->doms[i];
+    if (d.sensor >= t->nsensor || (d.sensor >= 0 && (d.acc < 0 || d.acc >= t->nacc)))
+      return fail(PPC_HIP_E_ARG, "DOM %d: sensor %d / acceptance %d out of range", i, d.sensor, d.acc);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (void* p : {(void*)c->d_sensors, (void*)c->d_pmt_dirs, (void*)c->d_rat, (void*)c->d_adoms, (void*)c->d_awvs})
+    if (p) (void)hipFree(p);
+  c->d_sensors = nullptr, c->d_pmt_dirs = nullptr, c->d_rat = nullptr, c->d_adoms = nullptr, c->d_awvs = nullptr;
+  c->accept_gsize = -1;
+  HIPCHK(dalloc(&c->d_sensors, std::max(t->nsensor, 1)));
+  HIPCHK(dalloc(&c->d_pmt_dirs, 3 * std::max(t->npmt, 1)));
+  HIPCHK(dalloc(&c->d_rat, (size_t)PPC_HIP_WNUM * std::max(t->nacc, 1)));
+  HIPCHK(dalloc(&c->d_adoms, std::max(t->gsize, 1)));
+  HIPCHK(dalloc(&c->d_awvs, 2 * PPC_HIP_WNUM));
+  if (t->nsensor) HIPCHK(hipMemcpy(c->d_sensors, t->sensors, sizeof(ppc_hip_sensor) * t->nsensor, hipMemcpyHostToDevice));
+  if (t->npmt) HIPCHK(hipMemcpy(c->d_pmt_dirs, t->pmt_dirs, sizeof(float) * 3 * t->npmt, hipMemcpyHostToDevice));
+  if (t->nacc) HIPCHK(hipMemcpy(c->d_rat, t->rat, sizeof(float) * PPC_HIP_WNUM * t->nacc, hipMemcpyHostToDevice));
+  if (t->gsize) HIPCHK(hipMemcpy(c->d_adoms, t->doms, sizeof(ppc_hip_dom_accept) * t->gsize, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_awvs, t->wvs, sizeof(float) * 2 * PPC_HIP_WNUM, hipMemcpyHostToDevice));
+  c->accept_gsize = t->gsize;
+  return PPC_HIP_OK;
+}
+
+int ppc_hip_accept(ppc_hip_ctx* c, uint64_t seed, ppc_hip_accepted* out, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (c->accept_gsize < 0) return fail(PPC_HIP_E_STATE, "no acceptance tables uploaded");
+  if (!c->launched) return fail(PPC_HIP_E_STATE, "acceptance without launch");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const uint32_t stored = std::min<uint32_t>(c->h_ctr->hidx, c->cfg.max_hits);
+  const uint32_t m = std::min(stored, cap);
+  if (m > 0) {
+    if (!c->d_accepted) HIPCHK(dalloc(&c->d_accepted, c->cfg.max_hits));
+    ppc::AcceptParams A{c->d_sensors, c->d_pmt_dirs, c->d_rat, c->d_adoms, c->d_awvs, c->accept_gsize};
+    HIPCHK(ppc::launch_accept(A, c->d_hits, c->d_hit_photon, m, seed, c->d_accepted, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out) HIPCHK(hipMemcpy(out, c->d_accepted, sizeof(ppc_hip_accepted) * m, hipMemcpyDeviceToHost));
+  }
+  *n = m;
   return PPC_HIP_OK;
 }
 

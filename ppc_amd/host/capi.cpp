@@ -191,4 +191,14 @@ int xppc_c_efin(void) {
   return 0;
 }
 
+const ppc_hip_accept_tables* xppc_c_accept_tables(void) { return g_init ? accept_tables() : nullptr; }
+
+int xppc_c_totals(uint64_t* photons, uint64_t* hits, double* device_ms) {
+  BackendInfo b = backend_info();
+  if (photons) *photons = b.photons;
+  if (hits) *hits = b.hits;
+  if (device_ms) *device_ms = b.device_ms;
+  return 0;
+}
+
 }  // extern "C"

@@ -64,6 +64,9 @@ struct Backend {
   unsigned pmax = 0, pmxo = 0, hnum = 0, hquo = 1;
   std::vector<ppc_hip_bunch> pz;
   std::vector<ppc_hip_hit> hits;
+  std::vector<ppc_hip_accepted> acc;  // device acceptance of hits (PPC_ACCEPT=device)
+  bool dev_accept = false;
+  unsigned long long launches = 0;
   unsigned hidx = 0;
   unsigned old = 0;
   double device_ms = 0;
@@ -108,6 +111,66 @@ bool deterministic() {
 }
 
 }  // namespace
+
+// the acceptance of print() on the device (PPC_ACCEPT=device): sensor types, acceptance
+// classes and per-DOM data in the layout of ppc_hip_accept_tables (valid until the next call)
+const ppc_hip_accept_tables* accept_tables() {
+  static std::vector<ppc_hip_sensor> sensors;
+  static std::vector<float> dirs, rat, wvs;
+  static std::vector<ppc_hip_dom_accept> doms;
+  static ppc_hip_accept_tables at;
+  sensors.clear(), dirs.clear(), rat.clear(), wvs.clear(), doms.clear();
+  std::map<int, int> sidx;
+  for (const auto& kv : det.types) {
+    const SensorType& S = kv.second;
+    ppc_hip_sensor x{};
+    x.def = S.def && S.beta != -2 ? 1 : 0;
+    x.beta = S.beta, x.fx = S.fx, x.Rr = S.Rr, x.Rz = S.Rz, x.cable = S.cable, x.mas = S.mas;
+    if (x.def) {
+      if (S.s.size() > 16) {
+        cerr << "Error: device acceptance takes at most 16 angular sensitivity coefficients" << endl;
+        exit(1);
+      }
+      x.ns = (int)S.s.size();
+      for (size_t k = 0; k < S.s.size(); k++) x.s[k] = S.s[k];
+    } else {
+      x.pmt0 = (int)(dirs.size() / 3), x.npmt = (int)S.dirs.size();
+      for (const Vec3& d : S.dirs) dirs.push_back(d[0]), dirs.push_back(d[1]), dirs.push_back(d[2]);
+    }
+    sidx[kv.first] = (int)sensors.size();
+    sensors.push_back(x);
+  }
+  std::map<std::pair<int, int>, int> aidx;
+  for (const auto& kv : det.accept)
+    if (kv.second.rde > 0 && (int)kv.second.rat.size() == WNUM) {
+      aidx[kv.first] = (int)(rat.size() / WNUM);
+      rat.insert(rat.end(), kv.second.rat.begin(), kv.second.rat.end());
+    }
+  doms.assign(det.t.gsize, ppc_hip_dom_accept{});
+  for (int i = 0; i < det.t.gsize; i++) {
+    const DomInfo& n = det.names[i];
+    ppc_hip_dom_accept& d = doms[i];
+    d.sensor = -1, d.acc = 0;
+    auto a = aidx.find(std::make_pair(n.omt, n.type));
+    auto t = sidx.find(n.omt);
+    if (n.hv > 0 && a != aidx.end() && t != sidx.end()) {
+      d.sensor = t->second, d.acc = a->second;
+      d.rde = n.rde / det.accept[a->first].rmax;
+    }
+    d.azi = n.azi;
+    for (int k = 0; k < 3; k++) d.tilt[k] = n.tilt[k];
+    d.R = det.oms[i].R, d.F = det.oms[i].F;
+  }
+  wvs.assign(2 * WNUM, 0.f);
+  for (int k = 0; k < WNUM; k++) wvs[2 * k] = det.wvs[k].i, wvs[2 * k + 1] = det.wvs[k].f;
+  at = ppc_hip_accept_tables{};
+  at.nsensor = (int)sensors.size(), at.sensors = sensors.data();
+  at.npmt = (int)(dirs.size() / 3), at.pmt_dirs = dirs.data();
+  at.nacc = (int)(rat.size() / WNUM), at.rat = rat.data();
+  at.gsize = det.t.gsize, at.doms = doms.data();
+  at.wvs = wvs.data();
+  return &at;
+}
 
 void print();
 
@@ -192,6 +255,15 @@ void ini() {  // gpu/ppc.cu:330-361 and gpu::ini 151-237
   check(ppc_hip_seed(be.ctx, sv), "seed");
   be.pz.assign(be.pmxo, ppc_hip_bunch{});
   be.hits.assign(be.hnum, ppc_hip_hit{});
+  {
+    const char* e = getenv("PPC_ACCEPT");
+    be.dev_accept = e != nullptr && std::string(e) == "device";
+  }
+  if (be.dev_accept) {
+    check(ppc_hip_upload_accept(be.ctx, accept_tables()), "acceptance tables");
+    be.acc.assign(be.hnum, ppc_hip_accepted{});
+    cerr << "Hit acceptance on the device" << endl;
+  }
   be.old = 0;
   pn = 0, pk = 0;
   cerr << "Photons per launch " << be.pmax << ", bunches " << be.pmxo << ", hits " << be.hnum << endl;
@@ -222,6 +294,11 @@ static void kernel(unsigned num) {
     uint32_t n = 0;
     check(ppc_hip_hits(be.ctx, be.hits.data(), nullptr, be.hnum, &n), "hits");
     be.hidx = n;
+    if (be.dev_accept) {  // before the next launch reuses the hit buffer
+      uint32_t m = 0;
+      check(ppc_hip_accept(be.ctx, ((unsigned long long)sv << 32) ^ be.launches, be.acc.data(), be.hnum, &m), "accept");
+      be.launches++;
+    }
     cerr << "photons: " << be.old << "  hits: " << be.hidx << endl;
   }
   if (num > 0) check(ppc_hip_launch(be.ctx, be.pz.data(), pk, num), "launch");
@@ -657,19 +734,53 @@ static bool hit_less_full(const ppc_hip_hit& a, const ppc_hip_hit& b) {  // DTMN
 void print() {
   ppc_hip_hit* hs = be.hits.data();
   unsigned nh = be.hidx;
-  if (deterministic())
+  const bool dev = be.dev_accept;
+  std::vector<uint32_t> order;  // device acceptance: sort an index, the records stay in hit order
+  if (dev) {
+    order.resize(nh);
+    for (unsigned i = 0; i < nh; i++) order[i] = i;
+    if (deterministic())
+      std::sort(order.begin(), order.end(), [hs](uint32_t a, uint32_t b) { return hit_less_full(hs[a], hs[b]); });
+    else if ((int)(flnb - flnd) < 0)
+      std::stable_sort(order.begin(), order.end(), [hs](uint32_t a, uint32_t b) { return (int)(hs[a].n - hs[b].n) < 0; });
+  } else if (deterministic())
     std::sort(hs, hs + nh, hit_less_full);
   else if ((int)(flnb - flnd) < 0)
     std::stable_sort(hs, hs + nh, [](const ppc_hip_hit& a, const ppc_hip_hit& b) { return (int)(a.n - b.n) < 0; });
 
-  for (unsigned i = 0; i < nh; i++) {
-    ppc_hip_hit& h = hs[i];
+  // one accepted photon: a HIT line or an entry of hitz (gpu/f2k.cxx:326-352)
+  auto output = [](const ppc_hip_hit& h, const DomInfo& n, int pmt, float wv, const float r[3], const float d[3]) {
+    if (cli_mode) {
+      if (nextgen)
+        printf("HIT %d %d_%d %f %f %f %f %f %f\n", n.str, n.dom, pmt, h.t, wv, h.pth, h.pph, h.dth, h.dph);
+      else
+        printf("HIT %d %d %f %f %f %f %f %f\n", n.str, n.dom, h.t, wv, h.pth, h.pph, h.dth, h.dph);
+    } else {
+      tmph.omkey = n;
+      tmph.pmt = pmt;
+      tmph.time = res(h.t, trz);
+      tmph.dir = res(1 + cos(h.pth), crz) - 1;
+      pout po;
+      po.r[0] = r[0], po.r[1] = r[1], po.r[2] = r[2], po.r[3] = h.t;
+      po.n[0] = d[0], po.n[1] = d[1], po.n[2] = d[2], po.n[3] = wv;
+      hitz[tmph].push_back(po);
+    }
+  };
+
+  for (unsigned q = 0; q < nh; q++) {
+    const unsigned idx = dev ? order[q] : q;
+    ppc_hip_hit& h = hs[idx];
     if ((int)(h.n - flnb) < 0 || (int)(h.n - flnd) > 0) {
       cerr << "Internal Error: " << h.n << " is not between " << flnb << " " << flnd << " " << flne << endl;
       continue;
     }
     for (; (int)(flnb - h.n) < 0; flnb++) pop_segment();
     const DomInfo& n = det.names[h.i];
+    if (dev) {
+      const ppc_hip_accepted& a = be.acc[idx];
+      if (a.pmt >= 0) output(h, n, a.pmt, a.wv, a.r, a.n);
+      continue;
+    }
     bool flag = n.hv > 0;
     if (flag) {
       Acceptance& mt = det.accept[std::make_pair(n.omt, n.type)];
@@ -717,22 +828,9 @@ void print() {
     if (flag) {
       const WvBin& wb = det.wvs[h.z];
       float wv = wb.i + (wb.f - wb.i) * xrnd();
-      if (cli_mode) {
-        if (nextgen)
-          printf("HIT %d %d_%d %f %f %f %f %f %f\n", n.str, n.dom, pmt, h.t, wv, h.pth, h.pph, h.dth, h.dph);
-        else
-          printf("HIT %d %d %f %f %f %f %f %f\n", n.str, n.dom, h.t, wv, h.pth, h.pph, h.dth, h.dph);
-      } else {
-        tmph.omkey = n;
-        tmph.pmt = pmt;
-        tmph.time = res(h.t, trz);
-        tmph.dir = res(1 + cos(h.pth), crz) - 1;
-        pout po;
-        float omr = det.oms[h.i].R;
-        po.r[0] = -omr * rx, po.r[1] = -omr * ry, po.r[2] = -omr * rz, po.r[3] = h.t;
-        po.n[0] = nx, po.n[1] = ny, po.n[2] = nz, po.n[3] = wv;
-        hitz[tmph].push_back(po);
-      }
+      float omr = det.oms[h.i].R;
+      const float r[3] = {-omr * rx, -omr * ry, -omr * rz}, d[3] = {nx, ny, nz};
+      output(h, n, pmt, wv, r, d);
     }
   }
   for (; (int)(flnb - flnd) < 0; flnb++) pop_segment();

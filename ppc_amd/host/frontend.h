@@ -26,6 +26,8 @@ struct BackendInfo {
   unsigned long long photons, hits, steps;
 };
 BackendInfo backend_info();
+// the tables of print()'s acceptance for the device (ppc_hip_upload_accept)
+const ppc_hip_accept_tables* accept_tables();
 
 }  // namespace xppc
 

@@ -1,0 +1,156 @@
+"""Hit post-processing on the device (ppc_hip_accept, PPC_ACCEPT=device) against
+the reference's host post-processing (print(), gpu/f2k.cxx:247-363).
+
+The device draws its random numbers from a counter-based generator, the host
+from the reference's stream, so the two agree statistically:
+  * per raw hit, the accepted photon's impact point and direction are a
+    deterministic function of the hit and must match the host formulas;
+  * the number of accepted hits must match the sum over hits of the acceptance
+    probability (relative DOM efficiency x angular sensitivity, cable shadow)
+    within 5 sigma of its binomial spread;
+  * through the library pipeline (flasher on the 7-string ice, deterministic
+    seeds, so both runs see the same raw hits), host and device acceptance give
+    per-DOM counts that agree (two-sample chi2 p > 0.01) and the same
+    wavelength distribution (KS p > 0.01).
+"""
+import ctypes
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy import stats
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WNUM = 512
+
+
+class Sensor(ctypes.Structure):
+    _fields_ = [("def_", ctypes.c_int32), ("ns", ctypes.c_int32), ("npmt", ctypes.c_int32), ("pmt0", ctypes.c_int32),
+                ("mas", ctypes.c_float), ("beta", ctypes.c_float), ("fx", ctypes.c_float), ("Rr", ctypes.c_float),
+                ("Rz", ctypes.c_float), ("cable", ctypes.c_float), ("s", ctypes.c_float * 16)]
+
+
+class DomAccept(ctypes.Structure):
+    _fields_ = [("sensor", ctypes.c_int32), ("acc", ctypes.c_int32), ("rde", ctypes.c_float), ("azi", ctypes.c_float),
+                ("tilt", ctypes.c_float * 3), ("R", ctypes.c_float), ("F", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+class AcceptTables(ctypes.Structure):
+    _fields_ = [("nsensor", ctypes.c_int32), ("sensors", ctypes.POINTER(Sensor)), ("npmt", ctypes.c_int32),
+                ("pmt_dirs", ctypes.POINTER(ctypes.c_float)), ("nacc", ctypes.c_int32),
+                ("rat", ctypes.POINTER(ctypes.c_float)), ("gsize", ctypes.c_int32),
+                ("doms", ctypes.POINTER(DomAccept)), ("wvs", ctypes.POINTER(ctypes.c_float))]
+
+
+def test_device_acceptance_matches_host_formulas(c1_dir):
+    from ppc_amd import api
+
+    det = api.Detector(c1_dir, {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+    T = AcceptTables.from_address(det.lib.xppc_c_accept_tables())
+    assert T.nsensor >= 1 and T.gsize == 420
+    dev = api.Device(0, deterministic=True)
+    dev.upload(det)
+    dev.upload_accept(det)
+    b = det.flasher_bunch(1, 30, num=40_000_000)
+    print("device acceptance: c1 flasher, %d photons" % b.num, flush=True)
+    dev.run([b])
+    h = dev.hits()
+    a = dev.accept(seed=12345)
+    # the same launch again: its hits stored in another order, post-processed identically
+    dev.seed(0)
+    dev.run([b])
+    h2 = dev.hits()
+    a2 = dev.accept(seed=12345)
+    dev.close()
+    key = lambda x: np.lexsort((x["dph"], x["dth"], x["pph"], x["pth"], x["z"], x["n"], x["t"], x["i"]))
+    o1, o2 = key(h), key(h2)
+    assert np.array_equal(h[o1], h2[o2]) and np.array_equal(a[o1], a2[o2])
+    assert len(a) == len(h) > 20000
+    ok = a["pmt"] >= 0
+    # geometry of the accepted photons (gpu/f2k.cxx:279-283, 344-351)
+    dom = np.array([(T.doms[i].R, T.doms[i].F) for i in range(T.gsize)], dtype=np.float32)
+    R, F = dom[h["i"], 0], dom[h["i"], 1]
+    f32 = np.float32
+    nx = np.sin(h["pth"]) * np.cos(h["pph"])
+    ny = np.sin(h["pth"]) * np.sin(h["pph"])
+    nz = np.cos(h["pth"])
+    dts = np.where(F < 0, f32(1), np.sin(h["dth"]))
+    rx, ry, rz = dts * np.cos(h["dph"]), dts * np.sin(h["dph"]), F * np.cos(h["dth"])
+    assert np.allclose(a["n"][ok], np.stack([nx, ny, nz], 1)[ok], atol=2e-6)
+    assert np.allclose(a["r"][ok], -R[ok, None] * np.stack([rx, ry, rz], 1)[ok], atol=2e-6)
+    # wavelengths inside their bins
+    wvs = np.ctypeslib.as_array(T.wvs, shape=(WNUM * 2,)).reshape(WNUM, 2)
+    lo, hi = wvs[h["z"], 0], wvs[h["z"], 1]
+    assert np.all((a["wv"][ok] >= lo[ok]) & (a["wv"][ok] <= hi[ok]))
+    # acceptance probability per hit: efficiency x angular sensitivity (one-PMT sensors here)
+    rat = np.ctypeslib.as_array(T.rat, shape=(T.nacc * WNUM,)).reshape(T.nacc, WNUM)
+    p = np.zeros(len(h))
+    for k, hit in enumerate(h):
+        d = T.doms[int(hit["i"])]
+        if d.sensor < 0:
+            continue
+        S = T.sensors[d.sensor]
+        assert S.def_ == 1 and d.azi < 0  # this ice: one PMT, no cable
+        pe = min(1.0, rat[d.acc, hit["z"]] * d.rde)
+        x = nx[k] * d.tilt[0] + ny[k] * d.tilt[1] + nz[k] * d.tilt[2]
+        poly = sum(S.s[j] * x ** j for j in range(S.ns))
+        pa = min(1.0, max(0.0, poly / S.mas)) if S.mas > 0 else float(rx[k] * d.tilt[0] + ry[k] * d.tilt[1]
+                                                                       + rz[k] * d.tilt[2] > S.s[0])
+        p[k] = pe * pa
+    want, sd = p.sum(), np.sqrt((p * (1 - p)).sum())
+    print("accepted %d of %d, expected %.1f +- %.1f" % (ok.sum(), len(h), want, sd), flush=True)
+    assert abs(ok.sum() - want) < 5 * sd + 1
+
+
+_LIB_RUN = r"""
+import json, os, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+from ppc_amd import api
+det = api.Detector(sys.argv[2], {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+det.start(0)
+det.lib.xppc_c_flset(1, 30)
+det.lib.xppc_c_sett(0.0, 0.0, 1.0, 1, 0, 0, 0.0)
+det.lib.xppc_c_flone(int(float(sys.argv[3])))
+det.lib.xppc_c_eout()
+h = det.hitz()
+p, raw, ms = det.totals()
+det.lib.xppc_c_efin()
+det.stop()
+print(json.dumps({"str": h["str"].tolist(), "dom": h["dom"].tolist(), "wv": h["wv"].tolist(), "raw": raw,
+                  "photons": p}))
+"""
+
+
+def test_library_device_acceptance_matches_host(c1_dir):
+    out = {}
+    for mode in ("host", "device"):  # one process each: the host draws advance the shared seed
+        env = dict(os.environ)
+        env.pop("PPC_ACCEPT", None)
+        if mode == "device":
+            env["PPC_ACCEPT"] = "device"
+        print("library pipeline, %s acceptance: c1 flasher" % mode, flush=True)
+        r = subprocess.run([sys.executable, "-c", _LIB_RUN, ROOT, c1_dir, "7.4e8"], capture_output=True, text=True,
+                           timeout=300, env=env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        out[mode] = json.loads(r.stdout.strip().splitlines()[-1])
+    ho, de = out["host"], out["device"]
+    assert ho["raw"] == de["raw"] > 20000 and ho["photons"] == de["photons"]  # the same raw hits
+    nh, nd = len(ho["dom"]), len(de["dom"])
+    print("accepted: host %d, device %d of %d raw hits" % (nh, nd, ho["raw"]), flush=True)
+    assert abs(nh - nd) < 5 * np.sqrt(nh + nd)
+    key = lambda o: np.array(o["str"]) * 1000 + np.array(o["dom"])
+    doms = np.union1d(key(ho), key(de))
+    ch = np.array([np.sum(key(ho) == d) for d in doms], dtype=float)
+    cd = np.array([np.sum(key(de) == d) for d in doms], dtype=float)
+    m = (ch + cd) >= 10
+    c2 = np.sum((ch[m] - cd[m]) ** 2 / (ch[m] + cd[m]))
+    pchi = stats.chi2.sf(c2, m.sum() - 1)
+    pks = stats.ks_2samp(ho["wv"], de["wv"]).pvalue
+    print("per-DOM chi2 p %.3f over %d DOMs, wavelength KS p %.3f" % (pchi, m.sum(), pks), flush=True)
+    assert pchi > 0.01 and pks > 0.01
