@@ -120,13 +120,13 @@ def gpu_run(args):
 
 def end_to_end_run():
     """the library-mode pipeline llh drives (xppc: sett + addp of the cascade, eout, hitz):
-    bunch upload, launches, hit download and the reference's host post-processing into hitz"""
+    bunch upload, launches, hit download and the hit post-processing into hitz, by wall
+    clock -- with the reference's host post-processing and with it on the device"""
     from ppc_amd import api, configs
 
     cas = configs.c2_cascade()
     det = api.Detector(configs.ensure("c2"), {"WFLA": None, "FWID": None, "DTMN": None})
     n = configs.f2k_direction(cas["th"], cas["ph"])
-    det.start(0)
     lib = det.lib
 
     def event():
@@ -137,16 +137,25 @@ def end_to_end_run():
         lib.xppc_c_efin()
         return k
 
-    event()  # warm-up
-    p0, h0, ms0 = det.totals()
-    t0 = time.perf_counter()
-    acc = event()
-    wall = time.perf_counter() - t0
-    p1, h1, ms1 = det.totals()
-    det.stop()
-    return {"workload": "one 100 TeV cascade through xppc::addp/eout into hitz (86-string ice)",
-            "photons": p1 - p0, "raw_hits": h1 - h0, "accepted_hits": acc, "wall_s": round(wall, 4),
-            "kernel_ms": round(ms1 - ms0, 2), "photons_per_s": round((p1 - p0) / wall, 1)}
+    out = {"workload": "one 100 TeV cascade through xppc::addp/eout into hitz (86-string ice)"}
+    for mode in ("host", "device"):
+        if mode == "device":
+            os.environ["PPC_ACCEPT"] = "device"
+        else:
+            os.environ.pop("PPC_ACCEPT", None)
+        det.start(0)
+        event()  # warm-up
+        p0, h0, ms0 = det.totals()
+        t0 = time.perf_counter()
+        acc = event()
+        wall = time.perf_counter() - t0
+        p1, h1, ms1 = det.totals()
+        det.stop()
+        out[mode + "_acceptance"] = {"photons": p1 - p0, "raw_hits": h1 - h0, "accepted_hits": acc,
+                                     "wall_s": round(wall, 4), "kernel_ms": round(ms1 - ms0, 2),
+                                     "photons_per_s": round((p1 - p0) / wall, 1)}
+    os.environ.pop("PPC_ACCEPT", None)
+    return out
 
 
 def adjoint_run(events=12, photons=1e8, seed=6):

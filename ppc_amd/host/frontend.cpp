@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <tuple>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -748,8 +749,11 @@ void print() {
   else if ((int)(flnb - flnd) < 0)
     std::stable_sort(hs, hs + nh, [](const ppc_hip_hit& a, const ppc_hip_hit& b) { return (int)(a.n - b.n) < 0; });
 
-  // one accepted photon: a HIT line or an entry of hitz (gpu/f2k.cxx:326-352)
-  auto output = [](const ppc_hip_hit& h, const DomInfo& n, int pmt, float wv, const float r[3], const float d[3]) {
+  // one accepted photon: a HIT line, or an entry of hitz (gpu/f2k.cxx:326-352), collected here
+  // and put into the map in key order below
+  std::vector<std::pair<ihit, pout>> batch;
+  auto output = [&batch](const ppc_hip_hit& h, const DomInfo& n, int pmt, float wv, const float r[3],
+                         const float d[3]) {
     if (cli_mode) {
       if (nextgen)
         printf("HIT %d %d_%d %f %f %f %f %f %f\n", n.str, n.dom, pmt, h.t, wv, h.pth, h.pph, h.dth, h.dph);
@@ -763,7 +767,7 @@ void print() {
       pout po;
       po.r[0] = r[0], po.r[1] = r[1], po.r[2] = r[2], po.r[3] = h.t;
       po.n[0] = d[0], po.n[1] = d[1], po.n[2] = d[2], po.n[3] = wv;
-      hitz[tmph].push_back(po);
+      batch.emplace_back(tmph, po);
     }
   };
 
@@ -835,6 +839,16 @@ void print() {
   }
   for (; (int)(flnb - flnd) < 0; flnb++) pop_segment();
   if (cli_mode) fflush(stdout);
+  // hitz gets the same entries, in the same order within an entry, as one insertion per
+  // photon would give; sorted by key, each insertion lands next to the previous one
+  std::stable_sort(batch.begin(), batch.end(),
+                   [](const std::pair<ihit, pout>& a, const std::pair<ihit, pout>& b) { return a.first < b.first; });
+  auto hint = batch.empty() ? hitz.end() : hitz.lower_bound(batch.front().first);
+  for (const auto& e : batch) {
+    auto it = hitz.emplace_hint(hint, std::piecewise_construct, std::forward_as_tuple(e.first), std::forward_as_tuple());
+    it->second.push_back(e.second);
+    hint = std::next(it);
+  }
 }
 
 // ------------------------------------------------------------ command-line sources
