@@ -209,6 +209,13 @@ int ppc_hip_lanes(const ppc_hip_ctx* ctx, int32_t* lanes);
 /* maximum bunches and hits per launch (getMaxBunchSize, gpu/ppc.cu:368) */
 int ppc_hip_capacity(const ppc_hip_ctx* ctx, uint32_t* max_bunches, uint32_t* max_hits);
 
+/* the clearance map the kernel uses to skip the DOM search on steps far from every string
+ * (no counterpart in the reference; host only): squares of side `step` over the strings'
+ * footprint + 100 m, origin (x0, y0), nx * ny entries (map == NULL: sizes only), each a
+ * lower bound of the horizontal distance from any point of the square to the nearest string
+ * cylinder (line radius, or hole-ice radius if larger) */
+int ppc_hip_clearance_map(const ppc_hip_tables* tables, float step, float* map, size_t cap, int32_t* nx, int32_t* ny,
+                          float* x0, float* y0);
 /* copy tables to the device (gpu::ini, gpu/ppc.cu:151-237) */
 int ppc_hip_upload(ppc_hip_ctx* ctx, const ppc_hip_tables* tables);
 /* reset every lane's MWC state to x = 362436069 + sv, c = 1234567 (rs_ini, gpu/ini.cxx:556-571) */

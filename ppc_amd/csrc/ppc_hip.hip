@@ -284,6 +284,34 @@ int ppc_hip_capacity(const ppc_hip_ctx* c, uint32_t* max_bunches, uint32_t* max_
   return PPC_HIP_OK;
 }
 
+int ppc_hip_clearance_map(const ppc_hip_tables* t, float step, float* map, size_t cap, int32_t* nx, int32_t* ny,
+                          float* x0, float* y0) {
+  if (!t || !nx || !ny || !x0 || !y0 || !(step > 0) || t->nsc < 1 || !t->sc)
+    return fail(PPC_HIP_E_ARG, "clearance map needs strings and a positive step");
+  float xl = t->sc[0].x, xh = xl, yl = t->sc[0].y, yh = yl;
+  for (int i = 1; i < t->nsc; i++) {
+    xl = std::min(xl, t->sc[i].x), xh = std::max(xh, t->sc[i].x);
+    yl = std::min(yl, t->sc[i].y), yh = std::max(yh, t->sc[i].y);
+  }
+  *x0 = xl - 100.f, *y0 = yl - 100.f;
+  *nx = (int)std::ceil((xh + 100.f - *x0) / step), *ny = (int)std::ceil((yh + 100.f - *y0) / step);
+  if (!map) return PPC_HIP_OK;
+  if (cap < (size_t)*nx * *ny) return fail(PPC_HIP_E_CAPACITY, "clearance map needs %d x %d entries", *nx, *ny);
+  // a point of the square is at most half a diagonal from its centre; 0.05 m covers rounding
+  const double slack = step * 0.70710678 + 0.05;
+  for (int j = 0; j < *ny; j++)
+    for (int i = 0; i < *nx; i++) {
+      const double cx = *x0 + (i + 0.5) * step, cy = *y0 + (j + 0.5) * step;
+      double best = 1e30;
+      for (int k = 0; k < t->nsc; k++) {
+        const ppc_hip_line& l = t->sc[k];
+        best = std::min(best, std::hypot(cx - l.x, cy - l.y) - std::max<double>(l.r, t->hr));
+      }
+      map[(size_t)j * *nx + i] = (float)(best - slack);
+    }
+  return PPC_HIP_OK;
+}
+
 int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   if (!c || !t) return fail(PPC_HIP_E_ARG, "NULL argument");
   if (t->size < 1 || t->size > PPC_HIP_MAXLYS) return fail(PPC_HIP_E_ARG, "layers %d out of range", t->size);
@@ -427,26 +455,10 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     float x0 = 0, y0 = 0;
     const float step = 4.f;
     if (t->nsc > 0 && !c->cfg.no_clearance) {
-      float xl = t->sc[0].x, xh = xl, yl = t->sc[0].y, yh = yl;
-      for (int i = 1; i < t->nsc; i++) {
-        xl = std::min(xl, t->sc[i].x), xh = std::max(xh, t->sc[i].x);
-        yl = std::min(yl, t->sc[i].y), yh = std::max(yh, t->sc[i].y);
-      }
-      x0 = xl - 100.f, y0 = yl - 100.f;
-      nx = (int)std::ceil((xh + 100.f - x0) / step), ny = (int)std::ceil((yh + 100.f - y0) / step);
+      const int need = ppc_hip_clearance_map(t, step, nullptr, 0, &nx, &ny, &x0, &y0);
+      if (need < 0) return need;
       cm.resize((size_t)nx * ny);
-      // a point of the square is at most half a diagonal from its centre; 0.05 m covers rounding
-      const double slack = step * 0.70710678 + 0.05;
-      for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) {
-          const double cx = x0 + (i + 0.5) * step, cy = y0 + (j + 0.5) * step;
-          double best = 1e30;
-          for (int k = 0; k < t->nsc; k++) {
-            const ppc_hip_line& l = t->sc[k];
-            best = std::min(best, std::hypot(cx - l.x, cy - l.y) - std::max<double>(l.r, t->hr));
-          }
-          cm[(size_t)j * nx + i] = (float)(best - slack);
-        }
+      ppc_hip_clearance_map(t, step, cm.data(), cm.size(), &nx, &ny, &x0, &y0);
     }
     if (c->d_clear) (void)hipFree(c->d_clear), c->d_clear = nullptr;
     if (!cm.empty()) {
