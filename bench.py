@@ -365,7 +365,8 @@ def main():
                                "100 TeV EM cascade, %.3g photons per step" % args.photons,
                    "photons_per_step": int(args.photons), "kernel_ms": round(g["kernel_ms"], 3),
                    "steps_per_photon": round(g["steps_per_photon"], 2),
-                   "hits_per_photon": g["hits_per_photon"], "flop_per_photon": round(g["flop_per_photon"], 1)},
+                   "hits_per_photon": g["hits_per_photon"], "flop_per_photon": round(g["flop_per_photon"], 1),
+                   "events_per_photon": {k: round(v, 4) for k, v in g["events_per_photon"].items()}},
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
     }
