@@ -40,6 +40,11 @@ __device__ __forceinline__ int rn(float x) { return (int)__builtin_rintf(x); }
 __device__ __forceinline__ int rd(float x) { return (int)__builtin_floorf(x); }
 __device__ __forceinline__ int ru(float x) { return (int)__builtin_ceilf(x); }
 
+// number of set bits of the wave mask m below this lane (v_mbcnt: no per-lane 64-bit mask)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // ---------------------------------------------------------------- RNG
 // Multiply-with-carry: t = a*x + c (one v_mad_u64_u32), x = lo(t), c = hi(t);
 // uniform in (0,1] from the top 23 bits of x (gpu/pro.cu:47-57).
@@ -481,7 +486,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;
   // per lane in memory: the photon's index, its segment and the MWC state it started from
   // (read back only for hits, records and the adjoint's second pass)
-  uint4* const info = P.lane_info + gid;
+  // (indexed by gid at each use rather than held as a per-lane 64-bit pointer)
   // ADJ: each record's photon runs twice from its start state: pass 0 sums the LLH
   // weights of its hits (Gsum), pass 1 scores the path with G, the weight of the hits
   // still to come
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         uint32_t i = 0xffffffffu;
         if (ADJ) {  // one record after the other per lane
           if (!live && pend) {  // the scoring pass of the photon just summed
-            const uint4 in = *info;
+            const uint4 in = P.lane_info[gid];
             s.x = in.z, s.c = in.w;
             i = in.x;
             G = Gsum, pass1 = true, pend = false;  // nw carries over from pass 0
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           if (!live) i = next_static, next_static += P.lanes;
           done = !__any(next_static < P.num);
         } else {
-          const uint32_t rank = __popcll(idle & ((1ull << lane) - 1));
+          const uint32_t rank = lanes_below(idle);
           const uint32_t avail = res_end - res_next;
           if (avail >= (uint32_t)nidle) {
             if (!live) i = res_next + rank;
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           ph.clear = 0.f;
           uint32_t seg;
           int fla = emit(P, i, s, ph, seg);
-          *info = make_uint4(i, seg, x0, c0);
+          P.lane_info[gid] = make_uint4(i, seg, x0, c0);
           ph.TOT = -flog(s());
           ph.SCA = 0;
           ph.old = fla;
@@ -810,7 +815,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       if (om == ofla)
         TOT = 0;
       else {
-        const uint32_t seg = info->y;
+        const uint32_t seg = P.lane_info[gid].y;
         if (seg < P.adj_nseg) {
           const float wh = P.adj_weight[(size_t)seg * P.gsize + om];
           if (!pass1)
@@ -828,7 +833,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         TOT = 0;
       else {
         ppc_hip_hit h;
-        const uint4 in = *info;
+        const uint4 in = P.lane_info[gid];
         h.i = om, h.t = r.w, h.n = in.y, h.z = ph.nhw >> 16;
         h.pth = acosf(n.z);
         h.pph = atan2f(n.y, n.x);
@@ -848,7 +853,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           first = atomicAdd(&P.ctr->hidx, (uint32_t)__popcll(hm));
         }
         first = __shfl(first, leader);
-        const uint32_t slot = first + __popcll(hm & ((1ull << lane) - 1));
+        const uint32_t slot = first + lanes_below(hm);
         if (slot < P.hnum) {
           P.hits[slot] = h;
           P.hit_photon[slot] = in.x;
@@ -873,10 +878,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         uint32_t first = 0;
         if ((int)lane == leader) first = atomicAdd(&P.ctr->nphot, (uint32_t)__popcll(em));
         first = __shfl(first, leader);
-        const uint32_t slot = first + __popcll(em & ((1ull << lane) - 1));
+        const uint32_t slot = first + lanes_below(em);
         if (slot < P.hit_photons_cap) {
           ppc_hip_adj_record rc;
-          const uint4 in = *info;
+          const uint4 in = P.lane_info[gid];
           rc.photon = in.x, rc.x0 = in.z, rc.c0 = in.w, rc.lane = gid;
           P.hit_photons[slot] = rc;
         }
