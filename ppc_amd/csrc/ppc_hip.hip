@@ -387,7 +387,16 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     if (bytes) memcpy(img.data() + off, src, bytes);
     return (int)off;
   };
-  P.off_az = put(t->az, sizeof(ppc_hip_aniz) * t->size);
+  {  // per layer (k1, k2, ra, rb), then (1/k1, 1/k2, 1/(k1 k2), k1 k2) for the scattering frame
+    std::vector<float4> az(2 * t->size);
+    for (int i = 0; i < t->size; i++) {
+      const ppc_hip_aniz& a = t->az[i];
+      const float okz = a.k1 * a.k2;
+      az[i] = make_float4(a.k1, a.k2, a.ra, a.rb);
+      az[t->size + i] = make_float4(1.f / a.k1, 1.f / a.k2, 1.f / okz, okz);
+    }
+    P.off_az = put(az.data(), sizeof(float4) * az.size());
+  }
   P.off_lr = put(t->lr, sizeof(float) * PPC_HIP_LMAX);
   P.off_mcol = put(t->mcol, sizeof(int32_t) * PPC_HIP_CTX * PPC_HIP_CTY);
   P.off_is = put(t->is, sizeof(uint16_t) * PPC_HIP_CX * PPC_HIP_CY);

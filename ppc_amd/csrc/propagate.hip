@@ -149,6 +149,7 @@ __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
 struct Tabs {
   const float4* lay;   // s0, s1, p, t
   const float4* az;    // k1, k2, ra, rb
+  const float4* azr;   // 1/k1, 1/k2, 1/(k1 k2), k1 k2
   const float* lr;
   const int* mcol;
   const uint16_t* is;
@@ -454,6 +455,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   Tabs T;
   T.lay = reinterpret_cast<const float4*>(base + P.off_lay);
   T.az = reinterpret_cast<const float4*>(base + P.off_az);
+  T.azr = T.az + P.size;
   T.lr = reinterpret_cast<const float*>(base + P.off_lr);
   T.mcol = reinterpret_cast<const int*>(base + P.off_mcol);
   T.is = reinterpret_cast<const uint16_t*>(base + P.off_is);
@@ -975,24 +977,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         xi = fminf(fmaxf(xi, 0.f), 2.f);
         if (ADJ && G != 0.f && !(EXT && hole))
           atomicAdd(&gacc[P.size + J], G * w.la * P.adj_ds1db[J] * frcp(w.sca(T, J)));
-        const float4 a = T.az[J];
-        const float k1 = a.x, k2 = a.y, okz = k1 * k2;
+        // (k1, k2, k1 k2) of the layer and their reciprocals, prepared on the host
+        const float4 a = T.az[J], ar = T.azr[J];
         const bool frame = !(EXT && hole);
         if (frame) {  // into the frame of anisotropic scattering
-          float n1 = (P.azx * n.x + P.azy * n.y) * k1;
-          float n2 = (-P.azy * n.x + P.azx * n.y) * k2;
+          float n1 = (P.azx * n.x + P.azy * n.y) * a.x;
+          float n2 = (-P.azy * n.x + P.azx * n.y) * a.y;
           n.x = n1 * P.azx - n2 * P.azy;
           n.y = n1 * P.azy + n2 * P.azx;
-          n.z = fdiv(n.z, okz);
+          n.z *= ar.z;
           normalize3(n.x, n.y, n.z);
         }
         rotate(xi, fsqrt(xi * (2 - xi)), n, s);
         if (frame) {  // and back
-          float n1 = fdiv(P.azx * n.x + P.azy * n.y, k1);
-          float n2 = fdiv(-P.azy * n.x + P.azx * n.y, k2);
+          float n1 = (P.azx * n.x + P.azy * n.y) * ar.x;
+          float n2 = (-P.azy * n.x + P.azx * n.y) * ar.y;
           n.x = n1 * P.azx - n2 * P.azy;
           n.y = n1 * P.azy + n2 * P.azx;
-          n.z *= okz;
+          n.z *= ar.w;
           normalize3(n.x, n.y, n.z);
         }
       }
