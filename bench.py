@@ -349,7 +349,7 @@ def main():
         else:
             cpu = cpu_port(cores, args.cpu_seconds)
     out = {
-        "metric": "photons/s on 1 GPU (86-string cascade) vs host CPU",
+        "metric": "photons/s on 1 GPU (86-string cascade) vs host CPU; LLH-gradient evals/s vs FD",
         "value": g["value"],
         "unit": "photons/s",
         "n_gpus": 1,

@@ -104,3 +104,31 @@ def test_cli_usage_without_device():
     r = subprocess.run([_native.ppc_exe()], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0
     assert "flasher" in r.stderr
+
+
+def test_c_abi_rejects_bad_arguments_without_a_device():
+    """argument checks come before any device work, and there is no CPU fallback:
+    without a GPU opening a context fails with a status and a message"""
+    from ppc_amd import api
+
+    lib = api.hip()
+    c_p = ctypes.c_void_p
+    n = ctypes.c_uint32()
+    assert lib.ppc_hip_open(0, None, None) < 0
+    assert lib.ppc_hip_upload(None, None) < 0
+    assert lib.ppc_hip_launch(None, None, 0, 0) < 0
+    assert lib.ppc_hip_wait(None, None) < 0
+    assert lib.ppc_hip_hits(None, None, None, 0, ctypes.byref(n)) < 0
+    assert lib.ppc_hip_set_layers(None, None, 1) < 0
+    assert lib.ppc_hip_graph_launch(None) < 0
+    assert lib.ppc_hip_accept(None, 0, None, 0, ctypes.byref(n)) < 0
+    assert lib.ppc_hip_upload_accept(None, None) < 0
+    assert lib.ppc_hip_adjoint(None, None, 0, 0, None, 0, None, 0, None, None, None, None) < 0
+    assert lib.ppc_hip_last_error()
+    import torch
+
+    if not torch.cuda.is_available():
+        ctx = c_p()
+        cfg = api.Config()
+        assert lib.ppc_hip_open(0, ctypes.byref(cfg), ctypes.byref(ctx)) < 0
+        assert b"" != lib.ppc_hip_last_error()
