@@ -126,9 +126,11 @@ __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
   float p2x = -nz * b, p2y = 0, p2z = nx * b;
   float qx = p1x - p2x, qy = p1y - p2y, qz = p1z - p2z;
   p2x += p1x, p2y += p1y, p2z += p1z;
-  float w = frsq(qx * qx + qy * qy + qz * qz);
+  // |p1| = |p2| = 1 and p1.p2 = ny nz a b, so |p1 - p2|^2 = 2 - cc and |p1 + p2|^2 = 2 + cc
+  const float cc = 2 * ny * nz * a * b;
+  float w = frsq(2 - cc);
   p1x = qx * w, p1y = qy * w, p1z = qz * w;
-  w = frsq(p2x * p2x + p2y * p2y + p2z * p2z);
+  w = frsq(2 + cc);
   p2x *= w, p2y *= w, p2z *= w;
   float u = s();
   float ps = __builtin_amdgcn_sinf(u), pc = __builtin_amdgcn_cosf(u);
