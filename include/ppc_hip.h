@@ -213,7 +213,8 @@ int ppc_hip_capacity(const ppc_hip_ctx* ctx, uint32_t* max_bunches, uint32_t* ma
  * (no counterpart in the reference; host only): squares of side `step` over the strings'
  * footprint + 100 m, origin (x0, y0), nx * ny entries (map == NULL: sizes only), each a
  * lower bound of the horizontal distance from any point of the square to the nearest string
- * cylinder (line radius, or hole-ice radius if larger) */
+ * cylinder (line radius, or hole-ice radius if larger), capped at 100 m; ppc_hip_upload
+ * builds it with squares of 1 m (2 m, 4 m, ... for footprints too large for 4M entries) */
 int ppc_hip_clearance_map(const ppc_hip_tables* tables, float step, float* map, size_t cap, int32_t* nx, int32_t* ny,
                           float* x0, float* y0);
 /* copy tables to the device (gpu::ini, gpu/ppc.cu:151-237) */

@@ -30,6 +30,11 @@ namespace {
 
 thread_local std::string g_err;
 
+// clearance map entries are capped at this distance (m); the kernel's map has at most
+// this many entries (16 MiB)
+constexpr double kClearCap = 100.0;
+constexpr size_t kClearMaxEntries = size_t(1) << 22;
+
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -297,18 +302,28 @@ int ppc_hip_clearance_map(const ppc_hip_tables* t, float step, float* map, size_
   *nx = (int)std::ceil((xh + 100.f - *x0) / step), *ny = (int)std::ceil((yh + 100.f - *y0) / step);
   if (!map) return PPC_HIP_OK;
   if (cap < (size_t)*nx * *ny) return fail(PPC_HIP_E_CAPACITY, "clearance map needs %d x %d entries", *nx, *ny);
-  // a point of the square is at most half a diagonal from its centre; 0.05 m covers rounding
+  // a point of the square is at most half a diagonal from its centre; 0.05 m covers rounding.
+  // Entries are capped at kClearCap (a smaller lower bound is still one), so each line only
+  // visits the squares whose centres lie within kClearCap of its cylinder.
   const double slack = step * 0.70710678 + 0.05;
-  for (int j = 0; j < *ny; j++)
-    for (int i = 0; i < *nx; i++) {
-      const double cx = *x0 + (i + 0.5) * step, cy = *y0 + (j + 0.5) * step;
-      double best = 1e30;
-      for (int k = 0; k < t->nsc; k++) {
-        const ppc_hip_line& l = t->sc[k];
-        best = std::min(best, std::hypot(cx - l.x, cy - l.y) - std::max<double>(l.r, t->hr));
+  std::vector<double> best((size_t)*nx * *ny, kClearCap);
+  for (int k = 0; k < t->nsc; k++) {
+    const ppc_hip_line& l = t->sc[k];
+    const double rad = std::max<double>(l.r, t->hr), reach = kClearCap + rad;
+    const int i0 = std::max(0, (int)std::floor((l.x - reach - *x0) / step));
+    const int i1 = std::min(*nx - 1, (int)std::ceil((l.x + reach - *x0) / step));
+    const int j0 = std::max(0, (int)std::floor((l.y - reach - *y0) / step));
+    const int j1 = std::min(*ny - 1, (int)std::ceil((l.y + reach - *y0) / step));
+    for (int j = j0; j <= j1; j++) {
+      const double dy = *y0 + (j + 0.5) * step - l.y;
+      for (int i = i0; i <= i1; i++) {
+        const double dx = *x0 + (i + 0.5) * step - l.x;
+        double& b = best[(size_t)j * *nx + i];
+        b = std::min(b, std::sqrt(dx * dx + dy * dy) - rad);
       }
-      map[(size_t)j * *nx + i] = (float)(best - slack);
     }
+  }
+  for (size_t q = 0; q < best.size(); q++) map[q] = (float)(best[q] - slack);
   return PPC_HIP_OK;
 }
 
@@ -458,13 +473,17 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     HIPCHK(hipMemcpy(c->d_domF, domF.data(), sizeof(float) * domF.size(), hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(c->d_rm, t->rm, sizeof(uint32_t) * c->lanes, hipMemcpyHostToDevice));
-  {  // clearance map of the string cylinders: squares of 4 m over the strings' footprint + 100 m
+  {  // clearance map of the string cylinders: squares of 1 m over the strings' footprint + 100 m
+     // (of 2 m, 4 m, ... where a map of 1 m squares would exceed kClearMaxEntries)
     std::vector<float> cm;
     int nx = 0, ny = 0;
     float x0 = 0, y0 = 0;
-    const float step = 4.f;
+    float step = 1.f;
     if (t->nsc > 0 && !c->cfg.no_clearance) {
-      const int need = ppc_hip_clearance_map(t, step, nullptr, 0, &nx, &ny, &x0, &y0);
+      int need;
+      while ((need = ppc_hip_clearance_map(t, step, nullptr, 0, &nx, &ny, &x0, &y0)) == PPC_HIP_OK &&
+             (size_t)nx * ny > kClearMaxEntries)
+        step *= 2;
       if (need < 0) return need;
       cm.resize((size_t)nx * ny);
       ppc_hip_clearance_map(t, step, cm.data(), cm.size(), &nx, &ny, &x0, &y0);
