@@ -412,7 +412,13 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     }
     P.off_az = put(az.data(), sizeof(float4) * az.size());
   }
-  P.off_lr = put(t->lr, sizeof(float) * PPC_HIP_LMAX);
+  {  // 1-d tilt reference distances, then the reciprocal of each interval (j-1, j)
+    std::vector<float> lr(2 * PPC_HIP_LMAX, 0.f);
+    for (int j = 0; j < PPC_HIP_LMAX; j++) lr[j] = t->lr[j];
+    for (int j = 1; j < t->lnum && j < PPC_HIP_LMAX; j++)
+      lr[PPC_HIP_LMAX + j] = 1.f / (t->lr[j] - t->lr[j - 1]);  // inf for a zero interval, as the division
+    P.off_lr = put(lr.data(), sizeof(float) * lr.size());
+  }
   P.off_mcol = put(t->mcol, sizeof(int32_t) * PPC_HIP_CTX * PPC_HIP_CTY);
   P.off_is = put(t->is, sizeof(uint16_t) * PPC_HIP_CX * PPC_HIP_CY);
   P.off_ls = put(t->ls, sizeof(uint16_t) * PPC_HIP_NSTR);

@@ -105,6 +105,9 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
 
 // Rotate unit n by the polar angle with 1-cos = cs, sin = si, around a uniform
 // azimuth (gpu/pro.cu:111-153); the largest component is swapped to x first.
+// NORM = false leaves out the final renormalisation, for callers whose next
+// operation is a linear map followed by a normalisation (which absorbs it).
+template <bool NORM = true>
 __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
   float nx = n.x, ny = n.y, nz = n.z;
   float r0 = nx * nx, r1 = ny * ny, r2 = nz * nz;
@@ -137,7 +140,7 @@ __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
   nx -= cs * nx - si * (pc * p1x + ps * p2x);
   ny -= cs * ny - si * (pc * p1y + ps * p2y);
   nz -= cs * nz - si * (pc * p1z + ps * p2z);
-  normalize3(nx, ny, nz);
+  if (NORM) normalize3(nx, ny, nz);
   if (ax == 1) {
     float t = nx; nx = ny; ny = t;
   } else if (ax == 2) {
@@ -152,7 +155,7 @@ struct Tabs {
   const float4* lay;   // s0, s1, p, t
   const float4* az;    // k1, k2, ra, rb
   const float4* azr;   // 1/k1, 1/k2, 1/(k1 k2), k1 k2
-  const float* lr;
+  const float* lr;     // [kLMAX] 1-d tilt reference distances, then [kLMAX] 1/(lr[j] - lr[j-1])
   const int* mcol;
   const uint16_t* is;
   const uint16_t* ls;
@@ -187,8 +190,8 @@ __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const f
     float z = (r.z - P.lmin) * P.lrdz;
     int i = j - 1, k = min(max(rd(z), 0), P.lpts);
     float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
-    float dw = T.lr[j] - T.lr[i];
-    float wx = fdiv(nr - T.lr[i], dw), wy = fdiv(T.lr[j] - nr, dw);
+    const float rdw = T.lr[kLMAX + j];  // 1 / (lr[j] - lr[j-1]), prepared on the host
+    float wx = (nr - T.lr[i]) * rdw, wy = (T.lr[j] - nr) * rdw;
     float qx = pj.x * wx + pi.x * wy;
     float qy = pj.y * wx + pi.y * wy;
     if (P.vthk != 0) dz = fdiv(dz, 1 - P.lrdz * qy);
@@ -616,8 +619,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           nr = fpow((B2 - nB) * An / 2, P.fr);
         }
         n1 *= P.k1, n2 *= P.k2, n3 *= P.kz;
-        nr *= frsq(n1 * n1 + n2 * n2 + n3 * n3);
-        TOT = fdiv(TOT, nr);
+        const float q = n1 * n1 + n2 * n2 + n3 * n3, rq = frsq(q);
+        if (EXT && P.fr != 0) {
+          nr *= rq;
+          TOT = fdiv(TOT, nr);
+        } else {
+          nr = rq;
+          TOT *= q * rq;  // 1/nr = sqrt(q) = q rsq(q)
+        }
       }
       float ahx = edh * (0.5f - sign * (z - I));
       J = I;
@@ -990,7 +999,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           n.z *= ar.z;
           normalize3(n.x, n.y, n.z);
         }
-        rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+        if (frame)  // the way back ends in a normalisation
+          rotate<false>(xi, fsqrt(xi * (2 - xi)), n, s);
+        else
+          rotate(xi, fsqrt(xi * (2 - xi)), n, s);
         if (frame) {  // and back
           float n1 = (P.azx * n.x + P.azy * n.y) * ar.x;
           float n2 = (-P.azy * n.x + P.azx * n.y) * ar.y;
