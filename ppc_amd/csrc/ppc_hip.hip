@@ -583,9 +583,10 @@ int stage(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant,
   c->h_P->nb = nb, c->h_P->num = num;
   c->h_P->bvar = variant ? c->d_bvar : nullptr;
   {  // claim chunks of photons per wave: large enough to keep the global counter cold, small
-     // enough that the last chunks of a launch still spread over all waves
+     // enough that the last chunks of a launch still spread over all waves: each wave
+     // makes about 128 claims per launch, within 64..1024 photons per claim
     const uint32_t waves = (uint32_t)c->lanes / 64;
-    uint64_t ch = num / ((uint64_t)waves * 16 + 1);
+    uint64_t ch = num / ((uint64_t)waves * 128 + 1);
     c->h_P->chunk = c->cfg.chunk ? c->cfg.chunk : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ch, 64), 1024);
   }
   return PPC_HIP_OK;
