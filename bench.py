@@ -268,8 +268,9 @@ def cpu_reference(cores, seconds):
     p, line = run(1e-6, 0)
     p.communicate(line, timeout=120)
     init = time.perf_counter() - t0
-    # ~1e5 photons/s per core: size the cascade for `seconds` of propagation per process
-    E = 1e5 * 1e5 * seconds / 1e9
+    # ~2.6e5 photons/s per core and ~1e4 photons per GeV of cascade: size the cascade for
+    # `seconds` of propagation per process
+    E = seconds * 2.6e5 / 1e4
     t0 = time.perf_counter()
     procs = [run(E, k) for k in range(cores)]
     for p, l in procs:
