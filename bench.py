@@ -25,9 +25,13 @@ The JSON line also carries
                 download and the host's hit post-processing, by wall clock;
   batch         (--batch) 512 sources of mixed types in 8 ice variants on the
                 7-string ice, one captured-graph replay per step, at the launch
-                shape the autotuner picks.
+                shape the autotuner picks;
+  full_feature_ice  (--ext) the same cascade (at most 2e8 photons per step) on
+                the 86-string ice with 2-d tilt, birefringence, hole ice and
+                both absorption anisotropies: the kernels with every option.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu] [--no-adjoint] [--batch]
+                    [--ext]
 """
 import argparse
 import json
@@ -321,6 +325,32 @@ print(b.num, time.perf_counter() - t0)
                       "(oracle/ppc_oracle.c)" % (cores, num)}
 
 
+def ext_run(photons, steps):
+    """the same cascade on the 86-string ice with every optional feature of the kernel on (2-d tilt,
+    birefringence, hole ice, per-layer anisotropy, old absorption anisotropy): the EXT kernels"""
+    from ppc_amd import api, configs
+
+    cas = configs.c2_cascade()
+    det = api.Detector(configs.ensure("c2x"), {"WFLA": None, "FWID": None, "DTMN": None})
+    n = configs.f2k_direction(cas["th"], cas["ph"])
+    b, mean = det.particle_bunch(cas["x"], cas["y"], cas["z"], cas["t"], cas["E"], 0.0, 11, n, num=int(photons))
+    dev = api.Device(0)
+    dev.upload(det)
+    dev.run([b])
+    ms, steps_total = [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = dev.run([b])
+        ms.append(s.device_ms)
+        steps_total += s.steps
+    wall = time.perf_counter() - t0
+    dev.close()
+    return {"workload": "100 TeV EM cascade, %.3g photons per step, on the 86-string ice with 2-d tilt, "
+                        "birefringence, hole ice and both absorption anisotropies" % photons,
+            "photons_per_s": round(photons * steps / wall, 1), "kernel_ms": round(sum(ms) / len(ms), 3),
+            "steps_per_photon": round(steps_total / (photons * steps), 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +361,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--ext", action="store_true", help="also time the full-feature ice (EXT kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagator runs on a single MI355X (one photon stream per lane, "
@@ -342,6 +373,7 @@ def main():
     e2e = None if args.no_adjoint else end_to_end_run()
     adj = None if args.no_adjoint else adjoint_run()
     bat = batch_run(args.steps) if args.batch else None
+    ext = ext_run(min(args.photons, 2e8), args.steps) if args.ext else None
     cpu = None
     if not args.no_cpu:
         cores = max(1, min(16, os.cpu_count() or 1))
@@ -377,6 +409,8 @@ def main():
         out["adjoint"] = adj
     if bat:
         out["batch"] = bat
+    if ext:
+        out["full_feature_ice"] = ext
     if cpu:
         out["config"]["gpu_over_cpu"] = round(g["value"] / cpu["value"], 1)
     print(json.dumps(out))
