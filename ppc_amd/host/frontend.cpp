@@ -224,9 +224,25 @@ void ini() {  // gpu/ppc.cu:330-361 and gpu::ini 151-237
     if (be.hquo < 1) be.hquo = 1;
     cerr << "HQUO(photons/max number of hits)=" << be.hquo << endl;
   }
-  unsigned npho = 1024;  // photons per lane per launch (NPHO, gpu/ini.cxx:21)
-  if (const char* e = getenv("NPHO"))
-    if (*e != 0 && atoi(e) > 0) npho = atoi(e);
+  // photons per lane per launch (NPHO, gpu/ini.cxx:21; the reference's default is 1024): with
+  // 458752 lanes, 512 makes launches of 2.3e8 photons, so that the host post-processing of
+  // one launch (print(), ~0.3 us per hit) finishes within the next launch's kernel time
+  unsigned npho = 512;
+  {  // NPHO_<device>, else NPHO (gpu/ppc.cu:120-131); a value <= 0 leaves no device to run on
+    const std::string key = "NPHO_" + std::to_string(be.device);
+    const char* e = getenv(key.c_str());
+    if (e == nullptr) e = getenv("NPHO");
+    if (e != nullptr && *e != 0) {
+      const int v = atoi(e);
+      cerr << "Setting NPHO=" << v << endl;
+      if (v <= 0) {
+        cerr << "Not using device # " << be.device << "!" << endl;
+        cerr << "No active GPU(s) selected!" << endl;
+        exit(5);
+      }
+      npho = (unsigned)v;
+    }
+  }
   unsigned long long pm = (unsigned long long)be.lanes * npho;
   if (pm > 0xffffff00ULL) pm = 0xffffff00ULL;
   be.pmax = (unsigned)pm;
