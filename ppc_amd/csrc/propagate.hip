@@ -1,18 +1,20 @@
 // propagate.hip -- photon propagation kernel for CDNA4 (gfx950).
 //
 // One persistent launch: every lane owns one multiply-with-carry stream and
-// propagates photons one after another.  Photons are handed out per wave: when
-// at least `refill` of the 64 lanes are idle, one lane claims that many photon
-// indices with a single atomic, and each idle lane emits its photon (source
-// sampling) before the wave steps again.  One loop iteration is one transport
+// propagates photons one after another.  Photons are handed out per wave: a wave
+// claims a chunk of photon indices from the global counter with one atomic, and
+// when at least `refill` of its 64 lanes are idle it gives them the next indices
+// of its chunk; each idle lane emits its photon (source sampling) before the wave
+// steps again.  One loop iteration is one transport
 // step of the reference kernel (gpu/pro.cu:486-889): absorption anisotropy,
 // layer walk to the next scattering/absorption point, DOM search through the
 // cell grid, hit recording, birefringence, and scattering.  Hits are compacted
 // per wave with ballot + prefix count into the HBM hit buffer.
 //
 // Physics and the order in which each photon consumes its MWC stream follow
-// gpu/pro.cu:287-904 line by line, so that a photon replayed from its recorded
-// stream state on the CPU gives the same path.  The per-wavelength ice tables
+// gpu/pro.cu:287-904 step by step and draw for draw, so that a photon replayed
+// from its recorded stream state on the CPU gives the same path (FP32 arithmetic
+// is rearranged where that only changes rounding).  The per-wavelength ice tables
 // are evaluated from per-layer factors held in LDS (sca = s0 + s1*la,
 // abs = p*lk + t*abl), instead of a 512 x layers table.
 #include <hip/hip_runtime.h>
