@@ -185,10 +185,16 @@ template <int TILT>
 __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const float4& r, float& dz) {
   if constexpr (TILT == 1) {
     float nr = P.lnx * r.x + P.lny * r.y - P.r0;
-    // first j >= 1 with nr < lr[j], else lnum - 1: a loop of uniform trip count over
-    // the reference points (scalar loads of lr), so lanes do not diverge
+    // first j >= 1 with nr < lr[j], else lnum - 1: uniform over the reference points
+    // (scalar loads of lr), so lanes do not diverge.  Candidates j = 4..1 are unrolled,
+    // so their values arrive together rather than one scalar load at a time (IceCube's
+    // tilt.par has 6 reference strings: 4 candidates); candidates above 4 take the loop
+    // first, so the assignments happen in the same order as one loop from lnum - 2 down.
     int j = P.lnum - 1;
-    for (int q = P.lnum - 2; q >= 1; q--) j = nr < P.lr[q] ? q : j;
+    for (int q = P.lnum - 2; q > 4; q--) j = nr < P.lr[q] ? q : j;
+#pragma unroll
+    for (int q = 4; q >= 1; q--)
+      if (q <= P.lnum - 2) j = nr < P.lr[q] ? q : j;
     float z = (r.z - P.lmin) * P.lrdz;
     int i = j - 1, k = min(max(rd(z), 0), P.lpts);
     float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
