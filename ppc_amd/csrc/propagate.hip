@@ -184,25 +184,29 @@ struct Wave {
 template <int TILT>
 __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const float4& r, float& dz) {
   if constexpr (TILT == 1) {
-    float nr = P.lnx * r.x + P.lny * r.y - P.r0;
-    // first j >= 1 with nr < lr[j], else lnum - 1: uniform over the reference points
-    // (scalar loads of lr), so lanes do not diverge.  Candidates j = 4..1 are unrolled,
-    // so their values arrive together rather than one scalar load at a time (IceCube's
-    // tilt.par has 6 reference strings: 4 candidates); candidates above 4 take the loop
-    // first, so the assignments happen in the same order as one loop from lnum - 2 down.
-    int j = P.lnum - 1;
-    for (int q = P.lnum - 2; q > 4; q--) j = nr < P.lr[q] ? q : j;
-#pragma unroll
-    for (int q = 4; q >= 1; q--)
-      if (q <= P.lnum - 2) j = nr < P.lr[q] ? q : j;
-    float z = (r.z - P.lmin) * P.lrdz;
-    int i = j - 1, k = min(max(rd(z), 0), P.lpts);
+    // the scalars of this branch read together, so that their loads share one wait
+    const float lnx = P.lnx, lny = P.lny, r0 = P.r0, lmin = P.lmin, lrdz = P.lrdz;
+    const int lnum = P.lnum, lpts = P.lpts;
+    const float lr1 = P.lr[1], lr2 = P.lr[2], lr3 = P.lr[3], lr4 = P.lr[4];
+    const float nr = lnx * r.x + lny * r.y - r0;
+    // first j >= 1 with nr < lr[j], else lnum - 1: uniform over the reference points, so
+    // lanes do not diverge.  Candidates above 4 take the loop, then j = 4..1 are tested
+    // from the values read above (IceCube's tilt.par has 6 reference strings: candidates
+    // 1..4), in the order of one loop from lnum - 2 down.
+    int j = lnum - 1;
+    for (int q = lnum - 2; q > 4; q--) j = nr < P.lr[q] ? q : j;
+    if (4 <= lnum - 2) j = nr < lr4 ? 4 : j;
+    if (3 <= lnum - 2) j = nr < lr3 ? 3 : j;
+    if (2 <= lnum - 2) j = nr < lr2 ? 2 : j;
+    if (1 <= lnum - 2) j = nr < lr1 ? 1 : j;
+    const float z = (r.z - lmin) * lrdz;
+    const int i = j - 1, k = min(max(rd(z), 0), lpts);
     float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
     const float rdw = T.lr[kLMAX + j];  // 1 / (lr[j] - lr[j-1]), prepared on the host
     float wx = (nr - T.lr[i]) * rdw, wy = (T.lr[j] - nr) * rdw;
     float qx = pj.x * wx + pi.x * wy;
     float qy = pj.y * wx + pi.y * wy;
-    if (P.vthk != 0) dz = fdiv(dz, 1 - P.lrdz * qy);
+    if (P.vthk != 0) dz = fdiv(dz, 1 - lrdz * qy);
     return qx + qy * (z - k);
   } else if constexpr (TILT == 2) {
     float qx = P.cb00 * r.x + P.cb10 * r.y;
@@ -680,9 +684,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       const float reach = sca * fsqrt(np);
       if (ph.clear <= reach) {
         ph.clear = 0.f;
-        const float gx = (r.x - P.cm_x0) * P.cm_rstep, gy = (r.y - P.cm_y0) * P.cm_rstep;
-        if (gx >= 0.f && gy >= 0.f && gx < (float)P.cm_nx && gy < (float)P.cm_ny)
-          ph.clear = P.clear_map[(int)gy * P.cm_nx + (int)gx];
+        // the map's parameters read together (one wait for their scalar loads), and the
+        // bounds tested without short-circuit branches
+        const float* cmap = P.clear_map;
+        const float x0 = P.cm_x0, y0 = P.cm_y0, rs = P.cm_rstep;
+        const int nx = P.cm_nx, ny = P.cm_ny;
+        const float gx = (r.x - x0) * rs, gy = (r.y - y0) * rs;
+        const bool in = (gx >= 0.f) & (gy >= 0.f) & (gx < (float)nx) & (gy < (float)ny);
+        if (in) ph.clear = cmap[(int)gy * nx + (int)gx];
       }
       if (ph.clear > reach) {
         search = false;
