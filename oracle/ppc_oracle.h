@@ -16,9 +16,15 @@
  *   gpu/ppc.cu:432-442  single-thread launch order    orc_run_xcpu
  *   (no counterpart)    score-function gradient       orc_adjoint_photon
  *
- * Pinning: every function is checked against the reference's own CPU build
- * (oracle/_ref, compiled from gpu/ppc.cu where it lies) photon by photon, on
- * random MWC states and bunches, and against closed forms (tests/test_oracle*.py).
+ * Pinning: every function but orc_adjoint_photon is checked against the
+ * reference's own CPU build (oracle/_ref, compiled from gpu/ppc.cu where it
+ * lies) photon by photon, on random MWC states and bunches, and against closed
+ * forms (tests/test_oracle*.py).  orc_adjoint_photon has no counterpart in the
+ * reference: its absorption and path-length terms are pinned by exact
+ * identities (the per-layer lengths sum to the path length from the hit time;
+ * straight paths give the per-layer moments, tests/test_adjoint_oracle.py); its
+ * scattering-event term (+la/(1-g)/sca at each scattering) is parity unpinned,
+ * checked only statistically against common-random-number differences.
  * The tables it consumes (orc_tables) are filled either from the reference
  * build (ref_export) or from the product's table builder; both are compared
  * field by field in the tests.
