@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <thread>
 #include <tuple>
 #include <iostream>
 #include <sstream>
@@ -748,6 +749,32 @@ static bool hit_less_full(const ppc_hip_hit& a, const ppc_hip_hit& b) {  // DTMN
 }
 
 // acceptance (RDE, angular sensitivity, PMT, cable shadow) and output of the raw hits (gpu/f2k.cxx:247-363)
+// stable sort of a launch's accepted photons by hitz key: chunks sorted on their own
+// threads, then adjacent chunks merged pairwise (both steps keep equal keys in order)
+static void sort_by_key(std::vector<std::pair<ihit, pout>>& b) {
+  auto less = [](const std::pair<ihit, pout>& x, const std::pair<ihit, pout>& y) { return x.first < y.first; };
+  const size_t n = b.size();
+  const unsigned T = std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 65536 || T < 2) {
+    std::stable_sort(b.begin(), b.end(), less);
+    return;
+  }
+  std::vector<size_t> cut(T + 1);
+  for (unsigned k = 0; k <= T; k++) cut[k] = n * k / T;
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < T; k++)
+    th.emplace_back([&b, &cut, &less, k] { std::stable_sort(b.begin() + cut[k], b.begin() + cut[k + 1], less); });
+  for (auto& t : th) t.join();
+  for (unsigned w = 1; w < T; w *= 2) {
+    th.clear();
+    for (unsigned k = 0; k + w < T; k += 2 * w) {
+      const size_t lo = cut[k], mid = cut[k + w], hi = cut[std::min(k + 2 * w, T)];
+      th.emplace_back([&b, &less, lo, mid, hi] { std::inplace_merge(b.begin() + lo, b.begin() + mid, b.begin() + hi, less); });
+    }
+    for (auto& t : th) t.join();
+  }
+}
+
 void print() {
   ppc_hip_hit* hs = be.hits.data();
   unsigned nh = be.hidx;
@@ -857,8 +884,7 @@ void print() {
   if (cli_mode) fflush(stdout);
   // hitz gets the same entries, in the same order within an entry, as one insertion per
   // photon would give; sorted by key, each insertion lands next to the previous one
-  std::stable_sort(batch.begin(), batch.end(),
-                   [](const std::pair<ihit, pout>& a, const std::pair<ihit, pout>& b) { return a.first < b.first; });
+  sort_by_key(batch);
   auto hint = batch.empty() ? hitz.end() : hitz.lower_bound(batch.front().first);
   for (const auto& e : batch) {
     auto it = hitz.emplace_hint(hint, std::piecewise_construct, std::forward_as_tuple(e.first), std::forward_as_tuple());
