@@ -751,7 +751,7 @@ static bool hit_less_full(const ppc_hip_hit& a, const ppc_hip_hit& b) {  // DTMN
 // acceptance (RDE, angular sensitivity, PMT, cable shadow) and output of the raw hits (gpu/f2k.cxx:247-363)
 // stable sort of a launch's accepted photons by hitz key: chunks sorted on their own
 // threads, then adjacent chunks merged pairwise (both steps keep equal keys in order)
-static void sort_by_key(std::vector<std::pair<ihit, pout>>& b) {
+void sort_by_key(std::vector<std::pair<ihit, pout>>& b) {
   auto less = [](const std::pair<ihit, pout>& x, const std::pair<ihit, pout>& y) { return x.first < y.first; };
   const size_t n = b.size();
   const unsigned T = std::min(8u, std::max(1u, std::thread::hardware_concurrency()));

@@ -28,6 +28,8 @@ struct BackendInfo {
 BackendInfo backend_info();
 // the tables of print()'s acceptance for the device (ppc_hip_upload_accept)
 const ppc_hip_accept_tables* accept_tables();
+// print()'s stable sort of accepted photons by hitz key (multi-threaded)
+void sort_by_key(std::vector<std::pair<ihit, pout>>& b);
 
 }  // namespace xppc
 

@@ -88,3 +88,53 @@ def test_muon_event_records():
     d = np.array(configs.f2k_direction(seg[0]["th"], seg[0]["ph"]))
     assert np.allclose((p1 - p0) / np.linalg.norm(p1 - p0), d, atol=1e-6)
     assert 0 <= seg[0]["th"] <= 80
+
+
+_SORT = r'''
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include <vector>
+#include "frontend.h"
+using namespace xppc;
+int main() {
+  std::mt19937 rng(3);
+  for (size_t n : {0ul, 1ul, 1000ul, 65535ul, 65536ul, 65537ul, 300001ul}) {
+    std::vector<std::pair<ihit, pout>> a(n);
+    for (size_t i = 0; i < n; i++) {  // few distinct keys: many ties, whose order must be kept
+      ihit& h = a[i].first;
+      h.track.first = 1, h.track.second = 2, h.track.frame = 0, h.track.t0 = 0;
+      h.omkey.str = 1 + rng() % 3, h.omkey.dom = 1 + rng() % 3;
+      h.pmt = rng() % 2, h.dir = 0.5f * (rng() % 2), h.time = (float)(rng() % 3);
+      a[i].second.r[0] = (float)i;
+    }
+    auto ref = a;
+    std::stable_sort(ref.begin(), ref.end(),
+                     [](const std::pair<ihit, pout>& x, const std::pair<ihit, pout>& y) { return x.first < y.first; });
+    sort_by_key(a);
+    for (size_t i = 0; i < n; i++)
+      if (a[i].second.r[0] != ref[i].second.r[0]) {
+        printf("n=%zu: position %zu differs\n", n, i);
+        return 1;
+      }
+  }
+  printf("ok\n");
+  return 0;
+}
+'''
+
+
+def test_hitz_key_sort_is_stable(tmp_path):
+    """print()'s multi-threaded sort of accepted photons by hitz key gives exactly
+    std::stable_sort's order (equal keys keep their hit order), so hitz entries are
+    filled in the order one insertion per photon would give."""
+    lib = os.path.join(ROOT, "ppc_amd", "lib")
+    src = tmp_path / "sort.cpp"
+    src.write_text(_SORT)
+    exe = str(tmp_path / "sort")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.join(ROOT, "ppc_amd", "host"), str(src), "-o", exe, "-L", lib, "-lxppc", "-pthread",
+                        "-Wl,-rpath," + lib], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
