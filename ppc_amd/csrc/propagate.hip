@@ -451,7 +451,8 @@ __device__ int emit(const KParams& P, uint32_t i, Mwc& s, Photon& ph, uint32_t& 
 // of this size would be held in SGPRs and spilled into VGPR lanes.
 // Occupancy: the kernel is VALU-issue bound with divergent branches, so more
 // waves per SIMD keep the VALU busier; at 7 waves the register budget is 72 VGPRs
-// (a few spills in the cold emission code), measured faster than 4, 5, 6 and 8.
+// (a few spills: reloaded at emission, after a DOM search and on the hit paths, not
+// on the plain step), measured faster than 4, 5, 6 (no spills) and 8.
 // ADJ: replay the photons of adjoint records (each from its recorded MWC state)
 // and accumulate d(LLH)/d(per-layer absorption, scattering) instead of hits:
 // along every path segment the score of the path density, -nr*d(abs)/da*l for
