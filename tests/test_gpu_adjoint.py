@@ -12,6 +12,8 @@ hit counts by per-layer absorption (adust400) and scattering (be400).
   * through torch.autograd: the backward of a Poisson log-likelihood equals the
     adjoint with weight k/mu - 1.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -91,7 +93,12 @@ def test_absorption_scores_sum_to_path_lengths_at_scale(env):
     assert sim.last_adjoint["missed"] == 0
 
 
-def _crn_check(env, which, direction, eps, repeats=24):
+# launches of one photon per lane per comparison; more (PPC_CRN_REPEATS=240) for a tighter
+# figure of the relative difference, printed with -s
+CRN_REPEATS = int(os.environ.get("PPC_CRN_REPEATS", "24"))
+
+
+def _crn_check(env, which, direction, eps, repeats=CRN_REPEATS):
     """LR directional derivative vs CRN central differences of sum_d w_d N_d.
 
     Every launch runs one photon per lane from chosen lane states, so photon k of
@@ -120,11 +127,15 @@ def _crn_check(env, which, direction, eps, repeats=24):
             m, _ = sim.forward(p, b0) if which == "a" else sim.forward(a0, p)
             val.append(np.sum(w * m))
         fd.append((val[0] - val[1]) / (2 * eps))
+    lanes = sim.dev.lanes
     sim.close()
     lr, fd = np.array(lr), np.array(fd)
     err = np.sqrt((lr.var(ddof=1) + fd.var(ddof=1)) / repeats)
-    print("%s: LR %.5g +- %.3g  FD %.5g +- %.3g" % (which, lr.mean(), lr.std(ddof=1) / np.sqrt(repeats), fd.mean(),
-                                                    fd.std(ddof=1) / np.sqrt(repeats)), flush=True)
+    d = lr - fd  # paired: both estimates from the same photons
+    print("%s: %d x %d photons: LR %.5g +- %.3g  FD %.5g +- %.3g  LR-FD relative %.2f%% +- %.2f%%" % (
+        which, repeats, lanes, lr.mean(), lr.std(ddof=1) / np.sqrt(repeats), fd.mean(),
+        fd.std(ddof=1) / np.sqrt(repeats), 100 * d.mean() / abs(fd.mean()),
+        100 * d.std(ddof=1) / np.sqrt(repeats) / abs(fd.mean())), flush=True)
     assert abs(lr.mean()) > 5 * err  # the comparison is informative
     assert abs(lr.mean() - fd.mean()) <= 4 * err + 0.02 * abs(fd.mean())
 
