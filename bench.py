@@ -28,10 +28,12 @@ The JSON line also carries
                 shape the autotuner picks;
   full_feature_ice  (--ext) the same cascade (at most 2e8 photons per step) on
                 the 86-string ice with 2-d tilt, birefringence, hole ice and
-                both absorption anisotropies: the kernels with every option.
+                both absorption anisotropies: the kernels with every option;
+  adjoint_86_string  (--adjoint-86) the gradient leg on the 86-string ice:
+                170 layers, 340 parameters.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu] [--no-adjoint] [--batch]
-                    [--ext]
+                    [--ext] [--adjoint-86]
 """
 import argparse
 import json
@@ -162,13 +164,17 @@ def end_to_end_run():
     return out
 
 
-def adjoint_run(events=12, photons=1e8, seed=6):
-    """C5: forward + adjoint of a Poisson LLH over 12 flasher events on the 7-string ice"""
+ICE_NAMES = {"c1": "7-string ice", "c2": "86-string ice (1-d tilt, absorption anisotropy)"}
+
+
+def adjoint_run(events=12, photons=1e8, seed=6, ice="c1"):
+    """C5: forward + adjoint of a Poisson LLH over 12 flasher events on the 7-string ice
+    (ice="c2": on the 86-string ice, 170 layers)"""
     import numpy as np
 
     from ppc_amd import adjoint, api, configs
 
-    det = api.Detector(configs.ensure("c1"), {"WFLA": "405", "FWID": "10", "DTMN": None})
+    det = api.Detector(configs.ensure(ice), {"WFLA": "405", "FWID": "10", "DTMN": None})
     rng = np.random.default_rng(seed)
     strs, doms = det.array("str", np.int32), det.array("dom", np.int32)
     pick = rng.choice(len(strs), events, replace=False)
@@ -178,7 +184,10 @@ def adjoint_run(events=12, photons=1e8, seed=6):
     # synthetic data: Poisson counts at an ice with every layer perturbed by up to 5 %
     mu_d, _ = sim.forward(sim.par_a * rng.uniform(0.95, 1.05, size), sim.par_b * rng.uniform(0.95, 1.05, size))
     k = rng.poisson(mu_d)
-    sim.forward(sim.par_a, sim.par_b)  # warm-up
+    # warm-up of both passes (first calls allocate the adjoint's buffers); an optimiser
+    # evaluates the gradient many times
+    mu0, launch0 = sim.forward(sim.par_a, sim.par_b)
+    sim.backward(launch0, np.where(mu0 > 0, k / np.maximum(mu0, 1e-9) - 1, -1.0))
     t0 = time.perf_counter()
     mu, launch = sim.forward(sim.par_a, sim.par_b)
     t1 = time.perf_counter()
@@ -188,8 +197,8 @@ def adjoint_run(events=12, photons=1e8, seed=6):
     sim.close()
     fwd, adj = t1 - t0, t2 - t1
     P = 2 * size
-    return {"workload": "%d flasher events x %.0e photons, 7-string ice, %d layers, %d parameters"
-                        % (events, photons, size, P),
+    return {"workload": "%d flasher events x %.0e photons, %s, %d layers, %d parameters"
+                        % (events, photons, ICE_NAMES[ice], size, P),
             "forward_s": round(fwd, 4), "forward_kernel_ms": round(launch["stats"].device_ms, 2),
             "adjoint_s": round(adj, 4), "adjoint_kernel_ms": round(sim.last_adjoint["ms"], 2),
             "replayed_photons": int(sim.last_adjoint["photons"]),
@@ -362,6 +371,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--ext", action="store_true", help="also time the full-feature ice (EXT kernels)")
+    ap.add_argument("--adjoint-86", action="store_true", help="also time the gradient on the 86-string ice")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagator runs on a single MI355X (one photon stream per lane, "
@@ -374,6 +384,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_run()
     bat = batch_run(args.steps) if args.batch else None
     ext = ext_run(min(args.photons, 2e8), args.steps) if args.ext else None
+    adj86 = adjoint_run(ice="c2") if args.adjoint_86 else None
     cpu = None
     if not args.no_cpu:
         cores = max(1, min(16, os.cpu_count() or 1))
@@ -411,6 +422,8 @@ def main():
         out["batch"] = bat
     if ext:
         out["full_feature_ice"] = ext
+    if adj86:
+        out["adjoint_86_string"] = adj86
     if cpu:
         out["config"]["gpu_over_cpu"] = round(g["value"] / cpu["value"], 1)
     print(json.dumps(out))
