@@ -167,21 +167,21 @@ def write_c1(path, seed=1, nrnd=131072):
     return path
 
 
-def c2_tilt_strings(strings):
-    """Six reference strings along the 225 deg tilt gradient (tilt.par)."""
+def c2_tilt_strings(strings, n=6):
+    """n reference strings (six by default) along the 225 deg tilt gradient (tilt.par)."""
     lnx, lny = math.cos(math.radians(225.0)), math.sin(math.radians(225.0))
     proj = [lnx * x + lny * y for x, y in strings]
     perp = [abs(-lny * x + lnx * y) for x, y in strings]
     cand = sorted(range(len(strings)), key=lambda k: (perp[k] > 70.0, proj[k]))
     near = [k for k in cand if perp[k] <= 70.0]
     near.sort(key=lambda k: proj[k])
-    pick_idx = np.linspace(0, len(near) - 1, 6).round().astype(int)
+    pick_idx = np.linspace(0, len(near) - 1, n).round().astype(int)
     pick = [near[i] for i in pick_idx]
     zero = min(pick, key=lambda k: abs(proj[k]))
     return [(k + 1, proj[k] - proj[zero]) for k in pick], zero + 1
 
 
-def write_c2(path, seed=2, nrnd=131072):
+def write_c2(path, seed=2, nrnd=131072, ntilt=6):
     os.makedirs(path, exist_ok=True)
     rng = np.random.default_rng(seed)
     strings = _string_positions(86)
@@ -196,17 +196,17 @@ def write_c2(path, seed=2, nrnd=131072):
     # f2k.cxx:502, ini.cxx:1488); eff is set so that the mean is 1e9.
     eff = 1e9 * 25.0 / (EM_YIELD * 1e5 * DPPM)
     _write_cfg(path, eff=eff, azimuth=azimuth, lk=(2.0 * lk / 3.0, -lk / 3.0, -lk / 3.0))
-    tilt, _ = c2_tilt_strings(strings)
+    tilt, _ = c2_tilt_strings(strings, ntilt)
     with open(os.path.join(path, "tilt.par"), "w") as f:
         for s, d in tilt:
             f.write("%d %.3f\n" % (s, d))
     rows = 1100.0 + 10.0 * np.arange(171)
-    phase = rng.uniform(0, 2 * math.pi, 6)
-    amp = rng.uniform(8.0, 20.0, 6)
+    phase = rng.uniform(0, 2 * math.pi, ntilt)
+    amp = rng.uniform(8.0, 20.0, ntilt)
     with open(os.path.join(path, "tilt.dat"), "w") as f:
         for dp in rows:
-            offs = [amp[k] * math.sin(2 * math.pi * (dp - 1100.0) / 850.0 + phase[k]) * (0.6 + 0.4 * k / 5.0)
-                    for k in range(6)]
+            offs = [amp[k] * math.sin(2 * math.pi * (dp - 1100.0) / 850.0 + phase[k]) *
+                    (0.6 + 0.4 * k / (ntilt - 1)) for k in range(ntilt)]
             f.write("%.2f " % dp + " ".join("%.4f" % o for o in offs) + "\n")
     _write_common(path, nrnd)
     return path
@@ -389,7 +389,13 @@ def f2k_lines(recs):
 F2K_TYPE = {"amu": 0, "brems": 11, "e": 11, "delta": 11, "epair": 11, "e-": 12, "e+": 13, "hadr": 101, "munu": 101}
 
 
-_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x}
+def write_c2t9(path, nrnd=131072):
+    """The 86-string ice with nine 1-d tilt reference strings instead of six (a tilt
+    table of more points than IceCube's tilt.par)."""
+    return write_c2(path, nrnd=nrnd, ntilt=9)
+
+
+_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9}
 
 
 def ensure(name, nrnd=131072):

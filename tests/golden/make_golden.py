@@ -41,6 +41,8 @@ CASES = {
     "c2": ("c2", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
     "c2x": ("c2x", {}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
     "c3": ("c2", {}, ("muon",)),
+    # the cascade of c2 on the 86-string ice with nine 1-d tilt reference strings
+    "c2t9": ("c2t9", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))

@@ -101,7 +101,8 @@ def _two_sample_chi2(a, b, min_exp=10):
     return chi2, dof, float(stats.chi2.sf(chi2, dof))
 
 
-SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000, "c3": 10}  # photons; c3: event repeats
+SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000, "c3": 10,  # photons; c3: event repeats
+           "c2t9": 100_000_000}
 
 
 @pytest.mark.parametrize("case", list(G.CASES))
