@@ -43,10 +43,16 @@ CASES = {
     "c3": ("c2", {}, ("muon",)),
     # the cascade of c2 on the 86-string ice with nine 1-d tilt reference strings
     "c2t9": ("c2t9", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
+    # flasher beams set by the environment (the bunch flset() builds for each)
+    "c1iso": ("c1", {"WFLA": "405", "FWID": "-1", "FLDR": "60"}, ("flasher_env", 1, 30)),
+    "c1lab": ("c1", {"WFLA": "405", "FWID": "1090", "FLDR": "30", "FTLT": "0.1 0.2 0.97"}, ("flasher_env", 1, 30)),
+    "c1cone": ("c1", {"WFLA": "405", "FWID": "-2", "ELEV": "20", "FZCR": "3", "OFLA": "0"}, ("flasher_env", 1, 30)),
+    # the c2 cascade with the tilt's layer-thickness correction
+    "c2vthk": ("c2", {"VTHK": "1"}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
-ENV_KEYS = ["WFLA", "FWID", "DTMN"]
+ENV_KEYS = ["WFLA", "FWID", "DTMN", "FLDR", "FTLT", "ELEV", "FZCR", "OFLA", "VTHK"]
 
 
 def _env(env):
@@ -63,6 +69,12 @@ def _bunch(T, src, num, det=None):
     the product's front-end (det) or, here, by the reference's own addp()"""
     if src[0] == "flasher":
         return U.flasher_bunch(T, src[1], src[2], num)
+    if src[0] == "flasher_env":  # flset() of the product's front-end (det) or of the reference
+        if det is not None:
+            return U.to_orc_bunch(det.flasher_bunch(src[1], src[2], num=num))
+        b = U.ref_flasher(src[1], src[2])
+        b.num = num
+        return b
     if src[0] == "muon":
         s = C3_SEGMENT
         n = np.array(s["n"]) / np.linalg.norm(s["n"])

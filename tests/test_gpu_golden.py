@@ -34,7 +34,7 @@ def _setup(case, **kw):
     from ppc_amd import api, configs
 
     cfg, env, src = G.CASES[case]
-    env = dict({"WFLA": None, "FWID": None}, **env)
+    env = dict({k: None for k in G.ENV_KEYS}, **env)
     env["DTMN"] = "1"
     det = api.Detector(configs.ensure(cfg), env)
     dev = api.Device(0, deterministic=True, **kw)
@@ -102,7 +102,8 @@ def _two_sample_chi2(a, b, min_exp=10):
 
 
 SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000, "c3": 10,  # photons; c3: event repeats
-           "c2t9": 100_000_000}
+           "c2t9": 100_000_000, "c1iso": 200_000_000, "c1lab": 200_000_000, "c1cone": 200_000_000,
+           "c2vthk": 100_000_000}
 
 
 @pytest.mark.parametrize("case", list(G.CASES))
@@ -121,7 +122,7 @@ def test_statistics_match_reference(case):
         N = sum(b.num for b in bunches)
     else:
         N = SOURCES[case]
-        bunches = [U.to_api_bunch(G._bunch(tab, src, N))]
+        bunches = [U.to_api_bunch(G._bunch(tab, src, N, det=det))]
     s = dev.run(bunches)
     assert s.photons == N
     hits = dev.hits()

@@ -72,7 +72,7 @@ print("photons_with_hits", len(np.unique(np.concatenate([hp, gp]))), ok)
 
 def _run(code, *args, env=None):
     e = dict(os.environ)
-    for k in ("WFLA", "FWID", "VTHK"):
+    for k in G.ENV_KEYS:
         e.pop(k, None)
     e.update(env or {})
     e["DTMN"] = "1"
