@@ -73,19 +73,21 @@ def _hex_sites(radius):
 def _string_positions(nstr):
     if nstr == 7:
         sites = _hex_sites(1)
-    else:
+    elif nstr <= 91:
         sites = _hex_sites(5)
+    else:
+        sites = _hex_sites(8)
     r = [math.hypot(x, y) for x, y in sites]
     ang = [math.degrees(math.atan2(y, x)) % 360.0 for x, y in sites]
     order = sorted(range(len(sites)), key=lambda k: (round(r[k], 3), round(ang[k], 3)))
     return [sites[k] for k in order[:nstr]]
 
 
-def _write_geo(path, strings):
+def _write_geo(path, strings, ndom=NDOM, step=DOM_STEP):
     with open(path, "w") as f:
         for s, (x, y) in enumerate(strings, start=1):
-            for d in range(1, NDOM + 1):
-                depth = DOM_DEPTH0 + DOM_STEP * (d - 1)
+            for d in range(1, ndom + 1):
+                depth = DOM_DEPTH0 + step * (d - 1)
                 omid = (s << 16) | d
                 f.write("S%03dD%02d\t0x%012x\t%.2f\t%.2f\t%.2f\t%d\t%d\n" % (s, d, omid, x, y, -depth, s, d))
 
@@ -389,13 +391,32 @@ def f2k_lines(recs):
 F2K_TYPE = {"amu": 0, "brems": 11, "e": 11, "delta": 11, "epair": 11, "e-": 12, "e+": 13, "hadr": 101, "munu": 101}
 
 
+def write_cbig(path, seed=4, nrnd=131072):
+    """A detector near the reference's capacity: 200 strings x 80 DOM positions at 12 m
+    spacing, 14280 of them in ice (on strings 1-86 the reference counts DOMs above 60 as
+    surface positions), 208 lines of at most 240 (strings 79-86 split), a cell grid at its
+    21 x 19 maximum, on the 170 layers of the 86-string ice without tilt or anisotropy."""
+    os.makedirs(path, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    strings = _string_positions(200)
+    depths = 1105.0 + 10.0 * np.arange(170)
+    be = np.exp(rng.uniform(math.log(0.02), math.log(0.1), 170))
+    ba = np.exp(rng.uniform(math.log(0.005), math.log(0.03), 170))
+    _write_geo(os.path.join(path, "geo-f2k"), strings, ndom=80, step=12.0)
+    _write_ice(path, depths, be, ba)
+    eff = 1e9 * 25.0 / (EM_YIELD * 1e5 * DPPM)
+    _write_cfg(path, eff=eff)
+    _write_common(path, nrnd)
+    return path
+
+
 def write_c2t9(path, nrnd=131072):
     """The 86-string ice with nine 1-d tilt reference strings instead of six (a tilt
     table of more points than IceCube's tilt.par)."""
     return write_c2(path, nrnd=nrnd, ntilt=9)
 
 
-_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9}
+_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9, "cbig": write_cbig}
 
 
 def ensure(name, nrnd=131072):

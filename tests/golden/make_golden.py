@@ -49,6 +49,8 @@ CASES = {
     "c1cone": ("c1", {"WFLA": "405", "FWID": "-2", "ELEV": "20", "FZCR": "3", "OFLA": "0"}, ("flasher_env", 1, 30)),
     # the c2 cascade with the tilt's layer-thickness correction
     "c2vthk": ("c2", {"VTHK": "1"}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
+    # a cascade in a detector near the reference's capacity (200 strings, 14280 DOMs)
+    "cbig": ("cbig", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
