@@ -51,10 +51,12 @@ CASES = {
     "c2vthk": ("c2", {"VTHK": "1"}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
     # a cascade in a detector near the reference's capacity (200 strings, 14280 DOMs)
     "cbig": ("cbig", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
+    # the full-feature ice with hole ice only for photons later than 300 ns and DOMs 3x oversized
+    "c2xho": ("c2x", {"HIFL": "300", "OVSZ": "3"}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
-ENV_KEYS = ["WFLA", "FWID", "DTMN", "FLDR", "FTLT", "ELEV", "FZCR", "OFLA", "VTHK"]
+ENV_KEYS = ["WFLA", "FWID", "DTMN", "FLDR", "FTLT", "ELEV", "FZCR", "OFLA", "VTHK", "HIFL", "OVSZ"]
 
 
 def _env(env):
