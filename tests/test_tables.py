@@ -52,10 +52,10 @@ for k in U.TABLE_FIELDS:
     if k in skip: continue
     a, b = bytes(R.field(k)), bytes(P.field(k))
     if a == b: continue
-    if k == "sca":
+    if k in ("sca", "abs_"):
         fa, fb = np.frombuffer(a, np.float32), np.frombuffer(b, np.float32)
         rel = np.max(np.abs(fa - fb) / np.maximum(np.abs(fa), 1e-30))
-        print("sca-maxrel", rel)
+        print("%s-maxrel" % k, rel)
         continue
     if k == "wvs_f":
         fa, fb = np.frombuffer(a, np.float32), np.frombuffer(b, np.float32)
@@ -73,10 +73,37 @@ def test_tables_match_reference(cfg):
 
     out = _child(_CMP, configs.ensure(cfg), HERE, env=ENVS[cfg])
     assert "DIFF" not in out, out
-    # the dense scattering table is computed in float with the same formula; the
-    # compilers may contract it differently (1 ulp in the synthetic ices)
+    # the dense scattering and absorption tables are computed in float with the same
+    # formulas, both built with fast-math: the compilers may contract and reassociate
+    # them differently (1 ulp in the synthetic ices; the absorption table's dust term,
+    # zero by default, is nonzero under ABSM=1)
     for line in out.splitlines():
-        if line.startswith("sca-maxrel"):
+        if line.split()[0].endswith("-maxrel"):
+            assert float(line.split()[1]) < 1e-6
+
+
+# table-level switches of the environment (gpu/ini.cxx:629-729, 829-845, 1359-1363, 1608-1640)
+ENV_SWITCHES = [{"BFRM": "1"}, {"BFRM": "2"}, {"BFRA": "0.8", "BFRB": "1.3"}, {"ABSM": "0"}, {"ABSM": "1"},
+                {"GECO": "1 30 0.5 -0.3"}]
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("env", ENV_SWITCHES, ids=lambda e: "-".join("%s=%s" % kv for kv in e.items()))
+@pytest.mark.parametrize("ice", ["c2x", "spice_bfr-v2"])
+def test_tables_match_reference_under_switches(ice, env):
+    """the same field-by-field comparison with the birefringence, absorption-table and
+    DOM-position switches of the environment set"""
+    if not U.ref_available():
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    from ppc_amd import configs
+
+    path = configs.ensure(ice) if ice.startswith("c") else os.path.join(REFICE, ice)
+    if not os.path.isdir(path):
+        pytest.skip("reference ice directories not present")
+    out = _child(_CMP, path, HERE, env=env)
+    assert "DIFF" not in out, out
+    for line in out.splitlines():
+        if line.split()[0].endswith("-maxrel"):
             assert float(line.split()[1]) < 1e-6
 
 
