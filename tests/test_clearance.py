@@ -16,8 +16,10 @@ class Line(ctypes.Structure):
                 ("dh", ctypes.c_float)]
 
 
-@pytest.mark.parametrize("name", ["c1", "c2"])
-def test_clearance_map_is_a_lower_bound(name):
+# squares of 1 m; the 200-string detector's footprint needs 2 m squares to stay under the
+# 4M entries the upload allows
+@pytest.mark.parametrize("name,step", [("c1", 1.0), ("c2", 1.0), ("cbig", 2.0)])
+def test_clearance_map_is_a_lower_bound(name, step):
     from ppc_amd import api, configs
 
     det = api.Detector(configs.ensure(name), {"WFLA": None, "FWID": None})
@@ -25,7 +27,6 @@ def test_clearance_map_is_a_lower_bound(name):
     lib.ppc_hip_clearance_map.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t] + \
         [ctypes.POINTER(ctypes.c_int32)] * 2 + [ctypes.POINTER(ctypes.c_float)] * 2
     nx, ny, x0, y0 = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float(), ctypes.c_float()
-    step = 1.0
     assert lib.ppc_hip_clearance_map(det.tables, step, None, 0, nx, ny, x0, y0) == 0
     cm = np.zeros(nx.value * ny.value, dtype=np.float32)
     assert lib.ppc_hip_clearance_map(det.tables, step, cm.ctypes.data_as(ctypes.c_void_p), cm.size, nx, ny, x0,
