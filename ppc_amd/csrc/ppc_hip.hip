@@ -206,10 +206,13 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
     delete c;
     return fail(PPC_HIP_E_ARG, "threads per workgroup must be a multiple of 64 up to 1024 (got %d)", k.threads);
   }
-  // 16 waves per CU (4 per SIMD: the kernel's register budget allows no more)
+  // 28 waves per CU (7 per SIMD: the occupancy the kernel's register budget is compiled for)
   if (k.blocks <= 0) k.blocks = c->cus * std::max(1, ppc::kWavesPerSimd * 4 * 64 / k.threads);
   if (k.refill <= 0) k.refill = 8;
-  if (k.chunk > 1024) return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);
+  if (k.chunk > 1024) {
+    delete c;
+    return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);
+  }
   if (k.refill > 64) k.refill = 64;
   if (k.max_bunches == 0) k.max_bunches = 1u << 20;
   if (k.max_hits == 0) k.max_hits = 1u << 24;
