@@ -96,3 +96,15 @@ def test_hit_buffer_overflow_keeps_what_fits(det):
     # the kept hits are hits of the same (deterministic) launch
     key = lambda a: set(map(bytes, a.view(np.dtype((np.void, a.dtype.itemsize)))))
     assert key(h) <= key(h_full)
+
+
+def test_bad_launch_shapes_are_rejected():
+    from ppc_amd import api
+
+    for kw in ({"threads": 100}, {"threads": 2048}, {"chunk": 2048}):
+        print("launch shape %s" % kw, flush=True)
+        with pytest.raises(api.PPCError):
+            api.Device(0, **kw)
+    dev = api.Device(0)  # and a good one still opens afterwards
+    assert dev.lanes > 0
+    dev.close()
