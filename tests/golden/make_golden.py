@@ -293,6 +293,34 @@ def make_cli(n=int(1.6e9)):
     print("cli c1: %d photons, %d HIT lines" % (photons, len(rows)))
 
 
+def cli_f2k_lines(events=8, E=1000.0):
+    """f2k TR lines of EM cascades on the 86-string ice (the benchmark cascade's position and
+    direction, energy E GeV): the input of the f2k mode of the command line"""
+    cas = configs.c2_cascade()
+    return ["TR 1 0 e %.3f %.3f %.3f %.4f %.4f 0 %.6g 0\n" % (cas["x"], cas["y"], cas["z"], cas["th"], cas["ph"], E)
+            for _ in range(events)]
+
+
+def make_cli_f2k():
+    """accepted HIT lines of the reference command line in its f2k mode (`ppc dev < f2k`),
+    one process per event, on the 86-string ice"""
+    env = _env({"PPCTABLESDIR": configs.ensure("c2")})
+    lines = cli_f2k_lines()
+    procs = [subprocess.Popen([U.REF_EXE, str(k)], env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True) for k in range(len(lines))]
+    outs, photons = [], 0
+    for p, l in zip(procs, lines):
+        out, err = p.communicate(l)
+        outs.append(out)
+        photons += sum(int(x.split()[1]) for x in err.splitlines() if x.startswith("photons:"))
+    rows = [l.split() for o in outs for l in o.splitlines() if l.startswith("HIT")]
+    arr = np.array([[float(v) for v in r[1:]] for r in rows], dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "cli_f2k_c2.npz"), photons=np.int64(photons),
+                        str=arr[:, 0].astype(np.int16), dom=arr[:, 1].astype(np.int16),
+                        t=arr[:, 2].astype(np.float32), wv=arr[:, 3].astype(np.float32))
+    print("cli f2k c2: %d photons, %d HIT lines" % (photons, len(rows)))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 5 and sys.argv[1] == "_worker":
         fn = {"stats": _stats_worker, "replay": _replay_worker}[sys.argv[2]]
@@ -316,3 +344,5 @@ if __name__ == "__main__":
             make_replay(c)
     if "cli" in what:
         make_cli()
+    if "cli" in what or "cli_f2k" in what:
+        make_cli_f2k()

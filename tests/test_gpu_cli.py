@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 from scipy import stats
 
-from test_gpu_golden import _two_sample_chi2
+from test_gpu_golden import G, _two_sample_chi2
 
 pytestmark = pytest.mark.gpu
 
@@ -35,8 +35,8 @@ def _env():
     return e
 
 
-def _compare(photons, strs, doms, t, wv):
-    gold = np.load(os.path.join(HERE, "golden", "cli_c1.npz"))
+def _compare(photons, strs, doms, t, wv, fixture="cli_c1.npz"):
+    gold = np.load(os.path.join(HERE, "golden", fixture))
     Nr, nr = int(gold["photons"]), len(gold["t"])
     z = (len(t) / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + len(t) / photons ** 2)
     p_rate = 2 * stats.norm.sf(abs(z))
@@ -64,6 +64,29 @@ def test_cli_flasher_matches_reference_cli():
     arr = np.array([[float(v) for v in row[1:5]] for row in rows])
     assert photons > 1e8
     _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3])
+
+
+def test_cli_f2k_stream_matches_reference_cli():
+    """`ppc 0 < events.f2k`: 1 TeV EM cascades (f2k TR lines) on the 86-string ice against the
+    reference command line's HIT lines on the same lines (tests/golden/cli_f2k_c2.npz, eight
+    events); ten times the reference's events here, one process"""
+    from ppc_amd import _native, configs
+
+    env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS}
+    env.update({"PPCTABLESDIR": configs.ensure("c2"), "DTMN": "1"})
+    lines = G.cli_f2k_lines(events=80)
+    print("f2k stream: %d TR lines on c2" % len(lines), flush=True)
+    r = subprocess.run([_native.ppc_exe(), "0"], env=env, input="".join(lines), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    photons = sum(int(m) for m in re.findall(r"photons: (\d+)", r.stderr))
+    out = r.stdout.splitlines()
+    # every input line is echoed, and the HIT lines follow the events
+    assert sum(l.startswith("TR ") for l in out) == len(lines)
+    rows = [l.split() for l in out if l.startswith("HIT ")]
+    arr = np.array([[float(v) for v in row[1:5]] for row in rows])
+    assert photons > 5e8
+    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_f2k_c2.npz")
 
 
 _LIB = r'''
