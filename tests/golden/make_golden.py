@@ -12,6 +12,8 @@ Fixtures (data only: inputs and the reference's outputs on them):
   replay_<case>.npz       single photons from random MWC states: state before emission,
                           state after transport, and every raw hit of each photon
   cli_c1.npz              HIT lines of the reference command line `ppc 1 30 N 0` (C1 flasher)
+  cli_f2k_c2.npz          HIT lines of `ppc k < TR line`, one 1 TeV EM cascade per process (86 strings)
+  cli_f2k_muon_c2.npz     the same for 1 km down-going 1 TeV muons (amu TR lines)
 The synthetic ice directories are regenerated from their seeds by ppc_amd.configs.
 """
 import ctypes
@@ -22,6 +24,7 @@ import tempfile
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -301,24 +304,32 @@ def cli_f2k_lines(events=8, E=1000.0):
             for _ in range(events)]
 
 
-def make_cli_f2k():
+def cli_f2k_muon_lines(events=32, E=1000.0, L=1000.0):
+    """f2k TR lines of down-going muons of energy E GeV and track length L m, starting L/2
+    above the benchmark cascade's position on the 86-string ice"""
+    cas = configs.c2_cascade()
+    return ["TR 1 0 amu %.3f %.3f %.3f 0 0 %.1f %.6g 0\n" % (cas["x"], cas["y"], cas["z"] + L / 2, L, E)
+            for _ in range(events)]
+
+
+def make_cli_f2k(name="cli_f2k_c2.npz", lines=None):
     """accepted HIT lines of the reference command line in its f2k mode (`ppc dev < f2k`),
     one process per event, on the 86-string ice"""
     env = _env({"PPCTABLESDIR": configs.ensure("c2")})
-    lines = cli_f2k_lines()
+    lines = lines or cli_f2k_lines()
     procs = [subprocess.Popen([U.REF_EXE, str(k)], env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
                               stderr=subprocess.PIPE, text=True) for k in range(len(lines))]
-    outs, photons = [], 0
-    for p, l in zip(procs, lines):
-        out, err = p.communicate(l)
-        outs.append(out)
-        photons += sum(int(x.split()[1]) for x in err.splitlines() if x.startswith("photons:"))
+    # one thread per process, so that the processes run side by side and none blocks on a full pipe
+    with ThreadPoolExecutor(len(procs)) as ex:
+        res = list(ex.map(lambda pl: pl[0].communicate(pl[1]), zip(procs, lines)))
+    outs = [out for out, _ in res]
+    photons = sum(int(x.split()[1]) for _, err in res for x in err.splitlines() if x.startswith("photons:"))
     rows = [l.split() for o in outs for l in o.splitlines() if l.startswith("HIT")]
     arr = np.array([[float(v) for v in r[1:]] for r in rows], dtype=np.float64)
-    np.savez_compressed(os.path.join(HERE, "cli_f2k_c2.npz"), photons=np.int64(photons),
+    np.savez_compressed(os.path.join(HERE, name), photons=np.int64(photons),
                         str=arr[:, 0].astype(np.int16), dom=arr[:, 1].astype(np.int16),
                         t=arr[:, 2].astype(np.float32), wv=arr[:, 3].astype(np.float32))
-    print("cli f2k c2: %d photons, %d HIT lines" % (photons, len(rows)))
+    print("%s: %d photons, %d HIT lines" % (name, photons, len(rows)))
 
 
 if __name__ == "__main__":
@@ -346,3 +357,5 @@ if __name__ == "__main__":
         make_cli()
     if "cli" in what or "cli_f2k" in what:
         make_cli_f2k()
+    if "cli" in what or "cli_f2k_muon" in what:
+        make_cli_f2k("cli_f2k_muon_c2.npz", cli_f2k_muon_lines())

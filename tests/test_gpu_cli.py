@@ -66,16 +66,21 @@ def test_cli_flasher_matches_reference_cli():
     _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3])
 
 
-def test_cli_f2k_stream_matches_reference_cli():
-    """`ppc 0 < events.f2k`: 1 TeV EM cascades (f2k TR lines) on the 86-string ice against the
-    reference command line's HIT lines on the same lines (tests/golden/cli_f2k_c2.npz, eight
-    events); ten times the reference's events here, one process"""
+@pytest.mark.parametrize("kind", ["cascade", "muon"])
+def test_cli_f2k_stream_matches_reference_cli(kind):
+    """`ppc 0 < events.f2k` on the 86-string ice against the reference command line's HIT lines
+    on the same TR lines: 1 TeV EM cascades (tests/golden/cli_f2k_c2.npz, 8 events) or 1 km
+    down-going 1 TeV muons (cli_f2k_muon_c2.npz, 32 events); ten times the reference's events
+    here, in one process"""
     from ppc_amd import _native, configs
 
     env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS}
     env.update({"PPCTABLESDIR": configs.ensure("c2"), "DTMN": "1"})
-    lines = G.cli_f2k_lines(events=80)
-    print("f2k stream: %d TR lines on c2" % len(lines), flush=True)
+    if kind == "cascade":
+        lines, fixture = G.cli_f2k_lines(events=80), "cli_f2k_c2.npz"
+    else:
+        lines, fixture = G.cli_f2k_muon_lines(events=320), "cli_f2k_muon_c2.npz"
+    print("f2k stream: %d %s TR lines on c2" % (len(lines), kind), flush=True)
     r = subprocess.run([_native.ppc_exe(), "0"], env=env, input="".join(lines), capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -86,7 +91,7 @@ def test_cli_f2k_stream_matches_reference_cli():
     rows = [l.split() for l in out if l.startswith("HIT ")]
     arr = np.array([[float(v) for v in row[1:5]] for row in rows])
     assert photons > 5e8
-    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_f2k_c2.npz")
+    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture=fixture)
 
 
 _LIB = r'''
