@@ -13,7 +13,7 @@ Fixtures (data only: inputs and the reference's outputs on them):
                           state after transport, and every raw hit of each photon
   cli_c1.npz              HIT lines of the reference command line `ppc 1 30 N 0` (C1 flasher)
   cli_f2k_c2.npz          HIT lines of `ppc k < TR line`, one 1 TeV EM cascade per process (86 strings)
-  cli_f2k_muon_c2.npz     the same for 1 km down-going 1 TeV muons (amu TR lines)
+  cli_f2k_muon_c2.npz     the same for 128 down-going 1 TeV muons of 1 km (amu TR lines)
 The synthetic ice directories are regenerated from their seeds by ppc_amd.configs.
 """
 import ctypes
@@ -304,7 +304,7 @@ def cli_f2k_lines(events=8, E=1000.0):
             for _ in range(events)]
 
 
-def cli_f2k_muon_lines(events=32, E=1000.0, L=1000.0):
+def cli_f2k_muon_lines(events=128, E=1000.0, L=1000.0):
     """f2k TR lines of down-going muons of energy E GeV and track length L m, starting L/2
     above the benchmark cascade's position on the 86-string ice"""
     cas = configs.c2_cascade()
