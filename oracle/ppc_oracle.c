@@ -854,6 +854,25 @@ int orc_adjoint_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_
   return nh;
 }
 
+/* The score of a photon's complete path: the same accumulation as the second run of
+ * orc_adjoint_photon with the weight held at 1 from emission to absorption (hits do not
+ * lower it).  Over complete paths the score has mean zero (the paths' density integrates
+ * to 1 for any ice), which pins the scattering-event term against the path-length term. */
+int orc_adjoint_path(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, const orc_rng* s0,
+                     orc_adjoint* A, orc_counters* cnt, uint32_t* row) {
+  orc_hit h[64];
+  orc_rng s = *s0;
+  orc_pbuf f;
+  orc_emit(t, pz, nb, idx, &s, &f);
+  if (row) *row = f.q;
+  const uint32_t nseg = A->nseg;
+  A->nseg = 0;
+  A->G = 1;
+  int nh = transport(t, &f, &s, h, 64, cnt, A);
+  A->nseg = nseg;
+  return nh;
+}
+
 uint32_t orc_run_xcpu(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t num, uint32_t nthr,
                       orc_rng* s, orc_hit* hits, uint32_t cap, orc_counters* cnt) {
   uint32_t nh = 0;

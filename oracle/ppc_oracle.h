@@ -14,17 +14,20 @@
  *   gpu/pro.cu:334-477  photon emission               orc_emit
  *   gpu/pro.cu:482-889  transport step loop           orc_transport
  *   gpu/ppc.cu:432-442  single-thread launch order    orc_run_xcpu
- *   (no counterpart)    score-function gradient       orc_adjoint_photon
+ *   (no counterpart)    score-function gradient       orc_adjoint_photon, orc_adjoint_path
  *
- * Pinning: every function but orc_adjoint_photon is checked against the
+ * Pinning: every function but the two gradient ones is checked against the
  * reference's own CPU build (oracle/_ref, compiled from gpu/ppc.cu where it
  * lies) photon by photon, on random MWC states and bunches, and against closed
- * forms (tests/test_oracle*.py).  orc_adjoint_photon has no counterpart in the
- * reference: its absorption and path-length terms are pinned by exact
+ * forms (tests/test_oracle*.py).  orc_adjoint_photon and orc_adjoint_path have no
+ * counterpart in the reference; they are pinned by what the path density fixes
+ * (tests/test_adjoint_oracle.py): the absorption and path-length scores by exact
  * identities (the per-layer lengths sum to the path length from the hit time;
- * straight paths give the per-layer moments, tests/test_adjoint_oracle.py); its
- * scattering-event term (+la/(1-g)/sca at each scattering) is parity unpinned,
- * checked only statistically against common-random-number differences.
+ * straight paths give the per-layer moments), the scattering-event term (+la/(1-g)/sca
+ * at each scattering) photon by photon (over a complete path the two scores of a layer
+ * give its scatterings as whole numbers that add up to the counted ones) and by the
+ * Poisson law of the scatterings (per layer E[n - sca l] = 0; in homogeneous ice
+ * E[L] = 1/a and Var[n - s L] = s/a).
  * The tables it consumes (orc_tables) are filled either from the reference
  * build (ref_export) or from the product's table builder; both are compared
  * field by field in the tests.
@@ -179,6 +182,10 @@ int orc_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, o
  * once accumulating the gradient; returns the number of hits */
 int orc_adjoint_photon(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, const orc_rng* s0,
                        orc_adjoint* A);
+/* the photon from state s0 with its complete path scored (weight 1 from emission to
+ * absorption, whatever it hits); row receives its wavelength row; returns its hits */
+int orc_adjoint_path(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t idx, const orc_rng* s0,
+                     orc_adjoint* A, orc_counters* cnt, uint32_t* row);
 /* the reference's single-thread launch: nthr threads share one MWC stream and
  * thread k emits then transports photons k, k+nthr, ... (gpu/ppc.cu:439-440) */
 uint32_t orc_run_xcpu(const orc_tables* t, const orc_bunch* pz, int nb, uint32_t num, uint32_t nthr,

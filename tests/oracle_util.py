@@ -95,6 +95,10 @@ def oracle():
         lib.orc_adjoint_photon.restype = ctypes.c_int
         lib.orc_adjoint_photon.argtypes = [ctypes.c_void_p, ctypes.POINTER(Bunch), ctypes.c_int, ctypes.c_uint32,
                                            ctypes.POINTER(Rng), ctypes.POINTER(Adjoint)]
+        lib.orc_adjoint_path.restype = ctypes.c_int
+        lib.orc_adjoint_path.argtypes = [ctypes.c_void_p, ctypes.POINTER(Bunch), ctypes.c_int, ctypes.c_uint32,
+                                         ctypes.POINTER(Rng), ctypes.POINTER(Adjoint), ctypes.POINTER(Counters),
+                                         ctypes.POINTER(ctypes.c_uint32)]
         _oracle = lib
     return _oracle
 
@@ -126,6 +130,35 @@ def oracle_adjoint(tables, bunches, photons, states, weight, lk, la, dpda, ds1db
         s = Rng(int(st[0]), int(st[1]), int(st[2]))
         nh[k] = lib.orc_adjoint_photon(tables, arr, len(bunches), int(i), ctypes.byref(s), ctypes.byref(A))
     return ga, gb, nh
+
+
+def oracle_path_scores(tables, bunch, states, lk, la, dpda, ds1db):
+    """Scores of complete photon paths (orc_adjoint_path), photon by photon: per photon the
+    per-layer absorption and scattering scores (float64 [n, size]), its wavelength row and
+    its number of scatterings."""
+    lib = oracle()
+    size = len(dpda)
+    f32 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    lk, la, dpda, ds1db = f32(lk), f32(la), f32(dpda), f32(ds1db)
+    weight = np.zeros((1, 1), dtype=np.float32)
+    n = len(states)
+    GA = np.zeros((n, size), dtype=np.float64)
+    GB = np.zeros((n, size), dtype=np.float64)
+    rows = np.zeros(n, dtype=np.uint32)
+    nsc = np.zeros(n, dtype=np.int64)
+    P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+    arr = (Bunch * 1)(bunch)
+    row = ctypes.c_uint32()
+    for k, st in enumerate(states):
+        ga, gb = GA[k], GB[k]
+        A = Adjoint(P(lk, ctypes.c_float), P(la, ctypes.c_float), P(dpda, ctypes.c_float), P(ds1db, ctypes.c_float),
+                    P(weight, ctypes.c_float), 0, 1, P(ga, ctypes.c_double), P(gb, ctypes.c_double), 0.0)
+        cnt = Counters()
+        s = Rng(int(st[0]), int(st[1]), int(st[2]))
+        lib.orc_adjoint_path(tables, arr, 1, k, ctypes.byref(s), ctypes.byref(A), ctypes.byref(cnt), ctypes.byref(row))
+        rows[k] = row.value
+        nsc[k] = cnt.scatters
+    return GA, GB, rows, nsc
 
 
 def point_bunch(x, y, z, num, t=0.0, seg=0):
