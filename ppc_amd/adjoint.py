@@ -20,7 +20,7 @@ the GPU (ppc_hip_adjoint) and accumulates in FP64.
     sim = HitSimulation(det, bunches, scale=1.0)
     a = torch.tensor(par_a, requires_grad=True); b = ...
     mu = ExpectedHits.apply(a, b, sim)          # [nseg, gsize]
-    llh = poisson_llh(mu, data)
+    llh = poisson_llh(mu, data)                 # or llf(mu, data).sum(), llh's own likelihood
     llh.backward()                              # a.grad, b.grad
 """
 import numpy as np
@@ -101,3 +101,77 @@ def poisson_llh(mu, k, floor=1e-9):
     """sum over bins of k log mu - mu (the hit-count log-likelihood up to a constant)"""
     k = torch.as_tensor(k, dtype=mu.dtype)
     return (k * torch.log(mu.clamp_min(floor)) - mu).sum()
+
+
+# model error of llh's likelihood: LSSL::SIG (llh/lssl.cxx:4), set from LSIG (llh/llh.cxx:2446-2448)
+LSIG = 0.10
+
+
+def _llf_solve(s, d, nsim, ndat, sig, tol=1e-8, max_iter=200):
+    """rates (ls, ld) per simulated and per data repetition that minimise
+    nsim ls - s log ls + ndat ld - d log ld + log(ld/ls)^2 / (2 sig^2): Newton's method on
+    ls with nsim ls + ndat ld = s + d (llh/llh.cxx:643-656), all bins at once"""
+    sd = s + d
+    ntot = nsim + ndat
+    ls = sd / ntot
+    ld = ls.clone()
+    if sig <= 0:
+        return ls, ld
+    s2 = sig * sig
+    act = sd > 0
+    for _ in range(max_iter):
+        if not bool(act.any()):
+            break
+        f = nsim * ls - s - torch.log(ld / ls) / s2
+        df = 1 + (1 / (ndat * ld) + 1 / (nsim * ls)) / s2
+        dls = torch.where(act, -f / (nsim * df), torch.zeros_like(ls))
+        ls = ls + dls
+        ld = torch.where(act, (sd - nsim * ls) / ndat, ld)
+        if bool(((ls < 0) | (ld < 0)).any()):
+            raise ValueError("llf: no solution for the rates (negative rate in Newton's method)")
+        act = act & (dls.abs() > tol * ls)
+    else:
+        if bool(act.any()):
+            raise ValueError("llf: Newton's method did not converge")
+    return ls, ld
+
+
+class _Llf(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, s, d, nsim, ndat, sig):
+        if bool((s < 0).any()) or bool((d < 0).any()):
+            raise ValueError("llf: negative counts")
+        ls, ld = _llf_solve(s, d, nsim, ndat, sig)
+        zero = torch.zeros_like(s)
+        llx = torch.log(ld / ls) ** 2 / (2 * sig * sig) if sig > 0 else zero
+        lls = torch.where(s > 0, s * torch.log((s / nsim) / ls), zero)
+        lld = torch.where(d > 0, d * torch.log((d / ndat) / ld), zero)
+        both0 = (s == 0) & (d == 0)
+        ctx.save_for_backward(s, d, ls, ld)
+        ctx.nsim, ctx.ndat = nsim, ndat
+        return torch.where(both0, zero, llx + lls + lld)
+
+    @staticmethod
+    def backward(ctx, g):
+        # at the minimising rates the value's derivative is the explicit one (envelope):
+        # d/ds = log((s/nsim)/ls), d/dd = log((d/ndat)/ld); a bin with no counts on a side
+        # has no finite derivative there and gets 0 (a bin without simulated hits has no
+        # photons for the adjoint to weight)
+        s, d, ls, ld = ctx.saved_tensors
+        zero = torch.zeros_like(s)
+        gs = torch.where(s > 0, torch.log((s / ctx.nsim) / ls.clamp_min(1e-300)), zero)
+        gd = torch.where(d > 0, torch.log((d / ctx.ndat) / ld.clamp_min(1e-300)), zero)
+        return g * gs, g * gd, None, None, None
+
+
+def llf(s, d, nsim=1.0, ndat=1.0, sig=None):
+    """llh's likelihood of simulated counts s (summed over nsim repetitions) against data
+    counts d (over ndat repetitions), bin by bin (llh/llh.cxx:637-662): the saturated
+    two-sample Poisson likelihood with a Gaussian model error sig on log(ld/ls) between
+    the rates of the two samples (sig = LSIG, 10 %; sig = 0: no model error).  Smaller is
+    better; 0 where both are 0.  Differentiable in s and d (float64)."""
+    sig = LSIG if sig is None else float(sig)
+    s = torch.as_tensor(s, dtype=torch.float64)
+    d = torch.as_tensor(d, dtype=torch.float64)
+    s, d = torch.broadcast_tensors(s, d)
+    return _Llf.apply(s, d, float(nsim), float(ndat), sig)
