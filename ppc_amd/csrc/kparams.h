@@ -1,10 +1,12 @@
 // kparams.h -- the parameter block of the propagation kernel.
 //
-// Scalars travel as a by-value kernel argument (uniform values land in SGPRs
-// through the kernel-argument segment); the small per-layer, per-string and
-// cell tables are packed by the host into one contiguous image that every
-// workgroup copies into LDS at start; the large tables (DOM positions, 2-D
-// tilt maps, wavelength factors) stay in HBM and are reached through L2.
+// The block lives in device memory and the kernel reads it through a
+// `const KParams*` argument (uniform fields become scalar loads, repeated where
+// the compiler needs them again; passed by value, its ~600 B would occupy SGPRs
+// the kernel needs).  The small per-layer, per-string and cell tables are packed
+// by the host into one contiguous image that every workgroup copies into LDS at
+// start; the large tables (DOM positions, tilt tables that do not fit in LDS,
+// wavelength factors, the clearance map) stay in HBM and are reached through L2.
 #ifndef PPC_KPARAMS_H
 #define PPC_KPARAMS_H
 

@@ -102,3 +102,73 @@ def test_scattering_mean_cosine(sf):
     if sf == 1.0:  # SAM: 1 - cos = 2 (1 - u^gr) -> P(cos <= c) = 1 - ((1+c)/2)^(1/gr)
         cdf = lambda c: ((1 + c) / 2) ** (1 / gr)
         assert stats.kstest(cos, cdf).pvalue > 1e-3
+
+
+def _homogeneous_c1(a0, s0):
+    """the 7-string ice with every layer and wavelength set to absorption a0, scattering s0"""
+    from ppc_amd import api, configs
+
+    det = api.Detector(configs.ensure("c1"), {"WFLA": None, "FWID": None})
+    T = U.product_tables(det)
+    for name, v in (("sca", s0), ("abs_", a0)):
+        arr = np.full(U.WNUM * U.MAXLYS, v, dtype=np.float32)
+        ctypes.memmove(T.field(name), arr.tobytes(), arr.nbytes)
+    wv = np.frombuffer(det.field("wv"), dtype=np.float32).reshape(U.WNUM, 8)
+    return det, T, wv
+
+
+def test_survival_in_homogeneous_non_scattering_ice():
+    """survival exp(-a L): without scattering, the path lengths to absorption (from the
+    absorption scores) follow Exp(a) (pro.cu:496, 546-572)"""
+    det, T, wv = _homogeneous_c1(0.03, 1e-7)
+    q = det.ice_parameters()
+    rm = det.array("rm", np.uint32)
+    rng = np.random.default_rng(8)
+    states = [(int(rng.integers(1, 2**32)), int(rng.integers(0, rm[k % 64])), int(rm[k % 64])) for k in range(8000)]
+    b = U.point_bunch(2.0, 1.0, 3.0, num=len(states))
+    GA, GB, rows, nsc = U.oracle_path_scores(T.ptr, b, states, wv[:, 4], wv[:, 3], q["dpda"], q["ds1db"])
+    L = (-GA / (wv[rows, 4][:, None].astype(np.float64) * q["dpda"][None, :].astype(np.float64))).sum(axis=1)
+    assert (nsc == 0).mean() > 0.999
+    assert stats.kstest(L, stats.expon(scale=1 / 0.03).cdf).pvalue > 1e-3
+    # a dropped or doubled absorption term moves the mean by a factor, far beyond its error
+    assert abs(L.mean() * 0.03 - 1) < 4 / math.sqrt(len(L))
+
+
+def test_hit_time_is_path_length_times_group_index():
+    """t = L n_g / c: in ice without scattering every hit lies on the straight line from the
+    source at path length t / ocm (ini.cxx:1693-1694), on the surface the DOM search
+    intersects: the DOM oversized by xR across the photon's direction only (pro.cu:634-651),
+    so with par the offset from the DOM centre along the direction and perp across it,
+    (par / R)^2 + (perp / (xR R))^2 = 1 and par <= 0 (the photon enters)"""
+    det, T, wv = _homogeneous_c1(0.003, 1e-7)
+    rm = det.array("rm", np.uint32)
+    oms = np.frombuffer(bytes(T.field("oms")), np.float32).reshape(-1, 5)
+    xR = float(np.frombuffer(bytes(T.field("xR")), np.float32)[0])
+    assert xR > 1
+    src = np.array([2.0, 1.0, 3.0])
+    b = U.point_bunch(*src, num=1)
+    lib = U.oracle()
+    H = (U.Hit * 64)()
+    rng = np.random.default_rng(9)
+    hits = []
+    for k in range(30000):
+        s = U.Rng(int(rng.integers(1, 2**32)), int(rng.integers(0, rm[k % 64])), int(rm[k % 64]))
+        m = lib.orc_photon(T.ptr, ctypes.byref(b), 1, 0, ctypes.byref(s), H, 64, None)
+        hits.append(U.hits_array(H, min(m, 64)))
+    h = np.concatenate(hits)
+    assert len(h) > 50
+    L = h["t"].astype(np.float64) / wv[h["z"], 0]
+    n = np.column_stack([np.sin(h["pth"]) * np.cos(h["pph"]), np.sin(h["pth"]) * np.sin(h["pph"]), np.cos(h["pth"])])
+    v = src[None, :] + L[:, None] * n - oms[h["i"]][:, 2:5].astype(np.float64)
+    par = np.sum(v * n, axis=1)
+    perp2 = np.sum(v * v, axis=1) - par ** 2
+    R = oms[h["i"], 0].astype(np.float64)
+    q = (par / R) ** 2 + perp2 / (xR * R) ** 2
+    # the direction comes back as FP32 angles: acos loses precision near the poles, and an
+    # angle error grows with the path into a position error; so the equation is tested where
+    # that error stays below 1e-4 m (paths under 150 m, |cos| < 0.95), the sign everywhere
+    sel = (L < 150) & (np.abs(np.cos(h["pth"])) < 0.95)
+    assert sel.sum() > 30
+    assert np.abs(q[sel] - 1).max() < 5e-3
+    assert np.abs(q - 1).max() < 0.05
+    assert (par <= 1e-3).all()
