@@ -23,6 +23,13 @@ static const float fcv = FPI / 180.f;
 
 static inline float sq(float x) { return x * x; }
 static inline float minf_(float a, float b) { return a < b ? a : b; }
+/* finite test on the bits: the oracle is built with --fast-math, under which isfinite() may
+ * be folded to true */
+static inline int finite_(float x) {
+  uint32_t u;
+  memcpy(&u, &x, sizeof u);
+  return (u & 0x7f800000u) != 0x7f800000u;
+}
 static inline float maxf_(float a, float b) { return a > b ? a : b; }
 static inline int mini_(int a, int b) { return a < b ? a : b; }
 static inline int maxi_(int a, int b) { return a > b ? a : b; }
@@ -693,6 +700,13 @@ static int transport(const orc_tables* t, const orc_pbuf* f, orc_rng* s, orc_hit
     r[1] += del * n[1];
     r[2] += del * n[2];
     r[3] += del * n[3];
+
+    /* the GPU build's guard against non-finite optical depths (pro.cu:727-729; the
+     * single-thread build has none, a NaN there never ends the photon) */
+    if (!finite_(TOT) || !finite_(SCA_)) {
+      TOT = 0, om = -1;
+      if (cnt) cnt->nonfinite++;
+    }
 
     if (om != -1) { /* DOM hit */
       if (om == ofla)

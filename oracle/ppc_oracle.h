@@ -13,6 +13,7 @@
  *   gpu/pro.cu:164-264  1-D / 2-D tilt                orc_zshift
  *   gpu/pro.cu:334-477  photon emission               orc_emit
  *   gpu/pro.cu:482-889  transport step loop           orc_transport
+ *                       (with the GPU build's non-finite guard, pro.cu:727-729)
  *   gpu/ppc.cu:432-442  single-thread launch order    orc_run_xcpu
  *   (no counterpart)    score-function gradient       orc_adjoint_photon, orc_adjoint_path
  *
@@ -141,6 +142,7 @@ typedef struct {
 
 typedef struct {
   uint64_t photons, steps, crossings, cells, strings, doms, scatters, hits, draws, killed;
+  uint64_t nonfinite; /* photons ended by the guard against non-finite optical depths */
 } orc_counters;
 
 /* gradient of the LLH weight of a photon's hits by per-layer absorption and

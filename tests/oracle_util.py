@@ -8,6 +8,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# the same restatement built keeping NaN semantics, for tests that force a NaN into a table
+ORACLE_NAN_SO = os.path.join(ROOT, "oracle", "liboracle_nan.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libppcref.so")
 REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ppc")
 
@@ -40,7 +42,7 @@ class Rng(ctypes.Structure):
 class Counters(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in
                 ("photons", "steps", "crossings", "cells", "strings", "doms", "scatters", "hits", "draws",
-                 "killed")]
+                 "killed", "nonfinite")]
 
 
 HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"),
@@ -62,16 +64,17 @@ def _setup(lib):
     return lib
 
 
-_oracle = None
+_oracle = {}
 _ref = None
 
 
-def oracle():
-    global _oracle
-    if _oracle is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError("oracle/liboracle.so is not built")
-        lib = _setup(ctypes.CDLL(ORACLE_SO))
+def oracle(nan=False):
+    """the oracle library; nan=True: its build that keeps NaN semantics"""
+    if nan not in _oracle:
+        path = ORACLE_NAN_SO if nan else ORACLE_SO
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built" % path)
+        lib = _setup(ctypes.CDLL(path))
         lib.orc_xrnd.restype = ctypes.c_float
         lib.orc_xrnd.argtypes = [ctypes.POINTER(Rng)]
         lib.orc_mrnd.restype = ctypes.c_float
@@ -99,8 +102,8 @@ def oracle():
         lib.orc_adjoint_path.argtypes = [ctypes.c_void_p, ctypes.POINTER(Bunch), ctypes.c_int, ctypes.c_uint32,
                                          ctypes.POINTER(Rng), ctypes.POINTER(Adjoint), ctypes.POINTER(Counters),
                                          ctypes.POINTER(ctypes.c_uint32)]
-        _oracle = lib
-    return _oracle
+        _oracle[nan] = lib
+    return _oracle[nan]
 
 
 class Adjoint(ctypes.Structure):
