@@ -529,19 +529,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
             s.x = in.z, s.c = in.w;
             i = in.x;
             G = Gsum, pass1 = true, pend = false;  // nw carries over from pass 0
-          } else if (!live) {
-            const uint32_t k = next_static;
+          } else if (!live && next_static < nwork) {
+            // (an exhausted lane's counter stays put: advancing it at every refill would
+            // wrap the 32-bit index around to work already done)
+            const ppc_hip_adj_record rc = P.adj_rec[next_static];
             next_static += P.lanes;
-            if (k < nwork) {
-              const ppc_hip_adj_record rc = P.adj_rec[k];
-              s.x = rc.x0, s.c = rc.c0, s.a = P.rm[rc.lane];
-              i = rc.photon;
-              G = 0.f, Gsum = 0.f, nw = 0, pass1 = false;
-            }
+            s.x = rc.x0, s.c = rc.c0, s.a = P.rm[rc.lane];
+            i = rc.photon;
+            G = 0.f, Gsum = 0.f, nw = 0, pass1 = false;
           }
           done = !__any(next_static < nwork || pend);
         } else if (P.deterministic) {
-          if (!live) i = next_static, next_static += P.lanes;
+          // photon next_static, then next_static + lanes, ...; a lane past the end keeps its
+          // counter (advancing it at every refill would wrap it around to photons already run)
+          if (!live && next_static < P.num) i = next_static, next_static += P.lanes;
           done = !__any(next_static < P.num);
         } else {
           const uint32_t rank = lanes_below(idle);

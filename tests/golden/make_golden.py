@@ -56,9 +56,20 @@ CASES = {
     "cbig": ("cbig", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
     # the full-feature ice with hole ice only for photons later than 300 ns and DOMs 3x oversized
     "c2xho": ("c2x", {"HIFL": "300", "OVSZ": "3"}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
+    # one whole 1 km down-going 1 TeV muon (the amu TR line of the command line's f2k stream test)
+    "c3km": ("c2", {}, ("muon_km",)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
+
+
+def c3km_track():
+    """the 1 km amu track of cli_f2k_muon_lines(): start L/2 above the benchmark cascade's
+    position, zenith 0 (down-going), 1 TeV"""
+    cas = configs.c2_cascade()
+    return dict(x=cas["x"], y=cas["y"], z=cas["z"] + 500.0, E=1000.0, dr=1000.0, n=configs.f2k_direction(0.0, 0.0))
+
+
 ENV_KEYS = ["WFLA", "FWID", "DTMN", "FLDR", "FTLT", "ELEV", "FZCR", "OFLA", "VTHK", "HIFL", "OVSZ"]
 
 
@@ -82,8 +93,8 @@ def _bunch(T, src, num, det=None):
         b = U.ref_flasher(src[1], src[2])
         b.num = num
         return b
-    if src[0] == "muon":
-        s = C3_SEGMENT
+    if src[0] in ("muon", "muon_km"):
+        s = C3_SEGMENT if src[0] == "muon" else c3km_track()
         n = np.array(s["n"]) / np.linalg.norm(s["n"])
         if det is not None:
             b, _ = det.particle_bunch(s["x"], s["y"], s["z"], 0.0, s["E"], s["dr"], 0, n, num=num)
@@ -182,6 +193,9 @@ def _stats_worker(args):
     return batches * pmax, np.concatenate(hits), int(T.get("gsize", np.int32)[0])
 
 
+# launches of 524288 photons per worker (8 workers) where the default of 8 is too few: the 1 km
+# track spreads its light over the whole string, so its per-DOM comparison needs more photons
+STATS_BATCHES = {"c3km": 32}
 STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 
@@ -347,10 +361,10 @@ if __name__ == "__main__":
             make_replay(c)
     if "stats" in what:
         for c in CASES:
-            make_stats(c)
+            make_stats(c, batches=STATS_BATCHES.get(c, 8))
     for c in CASES:
         if "stats_" + c in what:
-            make_stats(c)
+            make_stats(c, batches=STATS_BATCHES.get(c, 8))
         if "replay_" + c in what:
             make_replay(c)
     if "cli" in what:
