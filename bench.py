@@ -13,27 +13,28 @@ The JSON line also carries
                 counts of FLOPS below) over its measured duration, against the
                 MI355X FP32 vector peak;
   cpu_baseline  the reference's own CPU program (oracle/_ref/ppc, single
-                thread per process) on the same ice, one process per host core,
-                on a bounded cascade sample; the oracle restatement
-                ("port") where the reference program is not built;
-  adjoint       the gradient of a Poisson LLH of 12 flasher events (1e8 photons
-                each, 7-string ice, 100 layers: 200 parameters): forward and
-                adjoint time, their ratio, and the cost of the same gradient by
-                central differences (2 x 200 forward passes);
+                thread per process) on the same ice, one process per usable
+                host core (CPU affinity, bounded by the cgroup's CPU quota),
+                each propagating a cascade of at least 1e7 photons; the oracle
+                restatement ("port") where the reference program is not built;
+  adjoint       the gradient of llh's likelihood (llf) of 12 flasher events (1e8
+                photons each, 7-string ice, 100 layers: 200 parameters): forward
+                and adjoint time, their ratio, and the cost of the same gradient
+                by central differences (2 x 200 forward passes);
   end_to_end    the same cascade through the library-mode pipeline llh drives
                 (xppc::sett/addp/eout into hitz): bunch upload, launches, hit
                 download and the host's hit post-processing, by wall clock;
-  batch         (--batch) 512 sources of mixed types in 8 ice variants on the
-                7-string ice, one captured-graph replay per step, at the launch
-                shape the autotuner picks;
+  batch         512 sources of mixed types in 8 ice variants on the 7-string ice,
+                one captured-graph replay per step (at most 2 steps), at the
+                launch shape the autotuner picks (--no-batch skips it);
   full_feature_ice  (--ext) the same cascade (at most 2e8 photons per step) on
                 the 86-string ice with 2-d tilt, birefringence, hole ice and
                 both absorption anisotropies: the kernels with every option;
   adjoint_86_string  (--adjoint-86) the gradient leg on the 86-string ice:
                 170 layers, 340 parameters.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu] [--no-adjoint] [--batch]
-                    [--ext] [--adjoint-86]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--photons 1e9] [--no-cpu] [--cpu-photons 1e7]
+                    [--no-adjoint] [--no-batch] [--ext] [--adjoint-86]
 """
 import argparse
 import json
@@ -171,6 +172,7 @@ def adjoint_run(events=12, photons=1e8, seed=6, ice="c1"):
     """C5: forward + adjoint of a Poisson LLH over 12 flasher events on the 7-string ice
     (ice="c2": on the 86-string ice, 170 layers)"""
     import numpy as np
+    import torch
 
     from ppc_amd import adjoint, api, configs
 
@@ -183,21 +185,27 @@ def adjoint_run(events=12, photons=1e8, seed=6, ice="c1"):
     size = len(sim.par_a)
     # synthetic data: Poisson counts at an ice with every layer perturbed by up to 5 %
     mu_d, _ = sim.forward(sim.par_a * rng.uniform(0.95, 1.05, size), sim.par_b * rng.uniform(0.95, 1.05, size))
-    k = rng.poisson(mu_d)
+    k = torch.tensor(rng.poisson(mu_d), dtype=torch.float64)
+
+    def weights(mu):  # d llf / d mu: llh's likelihood of the data (llh/llh.cxx:637-662)
+        m = torch.tensor(mu, requires_grad=True)
+        adjoint.llf(m, k).sum().backward()
+        return m.grad.numpy()
+
     # warm-up of both passes (first calls allocate the adjoint's buffers); an optimiser
     # evaluates the gradient many times
     mu0, launch0 = sim.forward(sim.par_a, sim.par_b)
-    sim.backward(launch0, np.where(mu0 > 0, k / np.maximum(mu0, 1e-9) - 1, -1.0))
+    sim.backward(launch0, weights(mu0))
     t0 = time.perf_counter()
     mu, launch = sim.forward(sim.par_a, sim.par_b)
     t1 = time.perf_counter()
-    w = np.where(mu > 0, k / np.maximum(mu, 1e-9) - 1, -1.0)
+    w = weights(mu)
     ga, gb = sim.backward(launch, w)
     t2 = time.perf_counter()
     sim.close()
     fwd, adj = t1 - t0, t2 - t1
     P = 2 * size
-    return {"workload": "%d flasher events x %.0e photons, %s, %d layers, %d parameters"
+    return {"workload": "%d flasher events x %.0e photons, %s, %d layers, %d parameters; llh's likelihood llf"
                         % (events, photons, ICE_NAMES[ice], size, P),
             "forward_s": round(fwd, 4), "forward_kernel_ms": round(launch["stats"].device_ms, 2),
             "adjoint_s": round(adj, 4), "adjoint_kernel_ms": round(sim.last_adjoint["ms"], 2),
@@ -260,8 +268,31 @@ def batch_run(steps, seed=5):
             "tuned_ms_at_2e8": round(min(t for _, t in table), 2)}
 
 
-def cpu_reference(cores, seconds):
-    """the reference's CPU program, one process per core, cascade photons from stdin"""
+def usable_cores():
+    """host cores this process may run on: its CPU affinity, bounded by the CPU quota of its
+    cgroup where one is set (cgroup v2 cpu.max, v1 cfs quota/period)"""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        try:
+            q = float(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
+            per = float(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    if quota is not None:
+        n = min(n, max(1, int(quota)))
+    return n
+
+
+def cpu_reference(cores, photons_per_core):
+    """the reference's CPU program, one single-threaded process per core, each propagating one
+    cascade of about photons_per_core photons read from stdin"""
     from ppc_amd import configs
 
     exe = os.path.join(ROOT, "oracle", "_ref", "ppc")
@@ -281,27 +312,30 @@ def cpu_reference(cores, seconds):
     p, line = run(1e-6, 0)
     p.communicate(line, timeout=120)
     init = time.perf_counter() - t0
-    # ~2.6e5 photons/s per core and ~1e4 photons per GeV of cascade: size the cascade for
-    # `seconds` of propagation per process
-    E = seconds * 2.6e5 / 1e4
+    # an EM cascade yields about 1.0e4 photons per GeV on this ice (after the DOM-efficiency
+    # scaling of the front-end): 5 % more energy keeps each process above photons_per_core
+    E = 1.05 * photons_per_core / 1.0e4
     t0 = time.perf_counter()
     procs = [run(E, k) for k in range(cores)]
     for p, l in procs:
         p.stdin.write(l)
         p.stdin.close()
-    photons = 0
+    per_proc = []
     for p, _ in procs:
         err = p.stderr.read()
-        p.wait(timeout=600)
-        photons += sum(int(m) for m in re.findall(r"photons: (\d+)", err))
+        p.wait(timeout=1800)
+        per_proc.append(sum(int(m) for m in re.findall(r"photons: (\d+)", err)))
     wall = time.perf_counter() - t0 - init
+    photons = sum(per_proc)
     return {"value": photons / wall, "unit": "photons/s", "cores": cores, "kind": "reference",
-            "sample": "%d processes x one %.0f GeV EM cascade (%.3g photons in total) on the 86-string ice, "
-                      "reference CPU program gpu/ppc.cu built with -O2 --fast-math -DDTMN; %.1f s after %.2f s "
-                      "set-up" % (cores, E, photons, wall, init)}
+            "per_core": photons / wall / cores, "photons_per_core_min": min(per_proc),
+            "sample": "%d processes (one per usable host core) x one %.0f GeV EM cascade (%.3g photons in total, "
+                      "at least %.3g per process) on the 86-string ice, reference CPU program gpu/ppc.cu built "
+                      "with -O2 --fast-math -DDTMN; %.1f s after %.2f s set-up"
+                      % (cores, E, photons, min(per_proc), wall, init)}
 
 
-def cpu_port(cores, seconds):
+def cpu_port(cores, photons_per_core):
     """the oracle restatement (C), one process per core, when the reference program is absent"""
     code = r'''
 import ctypes, sys, time
@@ -321,7 +355,7 @@ t0 = time.perf_counter()
 U.oracle().orc_run_xcpu(T.ptr, ctypes.byref(b), 1, b.num, 512, ctypes.byref(s), H, 100000, None)
 print(b.num, time.perf_counter() - t0)
 '''
-    num = int(1e5 * seconds)
+    num = int(photons_per_core)
     procs = [subprocess.Popen([sys.executable, "-c", code, ROOT, str(num), str(k)], stdout=subprocess.PIPE,
                               stderr=subprocess.DEVNULL, text=True) for k in range(cores)]
     photons, wall = 0, 0.0
@@ -330,6 +364,7 @@ print(b.num, time.perf_counter() - t0)
         photons += int(out[0])
         wall = max(wall, float(out[1]))
     return {"value": photons / wall, "unit": "photons/s", "cores": cores, "kind": "port",
+            "per_core": photons / wall / cores,
             "sample": "%d processes x %d photons of the 100 TeV cascade on the 86-string ice, oracle restatement "
                       "(oracle/ppc_oracle.c)" % (cores, num)}
 
@@ -367,9 +402,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e9)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-photons", type=float, default=1e7, help="photons per host core of the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--ext", action="store_true", help="also time the full-feature ice (EXT kernels)")
     ap.add_argument("--adjoint-86", action="store_true", help="also time the gradient on the 86-string ice")
     args = ap.parse_args()
@@ -382,16 +417,16 @@ def main():
     g = gpu_run(args)
     e2e = None if args.no_adjoint else end_to_end_run()
     adj = None if args.no_adjoint else adjoint_run()
-    bat = batch_run(args.steps) if args.batch else None
+    bat = None if args.no_batch else batch_run(min(args.steps, 2))
     ext = ext_run(min(args.photons, 2e8), args.steps) if args.ext else None
     adj86 = adjoint_run(ice="c2") if args.adjoint_86 else None
     cpu = None
     if not args.no_cpu:
-        cores = max(1, min(16, os.cpu_count() or 1))
+        cores = usable_cores()
         if os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ppc")):
-            cpu = cpu_reference(cores, args.cpu_seconds)
+            cpu = cpu_reference(cores, args.cpu_photons)
         else:
-            cpu = cpu_port(cores, args.cpu_seconds)
+            cpu = cpu_port(cores, args.cpu_photons)
     out = {
         "metric": "photons/s on 1 GPU (86-string cascade) vs host CPU; LLH-gradient evals/s vs FD",
         "value": g["value"],

@@ -221,6 +221,13 @@ int ppc_hip_clearance_map(const ppc_hip_tables* tables, float step, float* map, 
 int ppc_hip_upload(ppc_hip_ctx* ctx, const ppc_hip_tables* tables);
 /* reset every lane's MWC state to x = 362436069 + sv, c = 1234567 (rs_ini, gpu/ini.cxx:556-571) */
 int ppc_hip_seed(ppc_hip_ctx* ctx, uint32_t sv);
+/* seed != 0: in the launches that follow, photon i starts from an MWC state of its own, a
+ * hash of (seed, i), on the multiplier of the lane that runs it (lane i % lanes in
+ * deterministic launches), instead of continuing its lane's stream; launches with the same
+ * seed then use the same random numbers photon by photon (common random numbers for
+ * differences between ice models).  0 (default): lanes continue their streams.  No
+ * counterpart in the reference. */
+int ppc_hip_seed_photons(ppc_hip_ctx* ctx, uint64_t seed);
 /* read/write the per-lane MWC states (x | c << 32), lanes entries */
 int ppc_hip_get_states(ppc_hip_ctx* ctx, uint64_t* states, int32_t lanes);
 int ppc_hip_set_states(ppc_hip_ctx* ctx, const uint64_t* states, int32_t lanes);

@@ -54,6 +54,15 @@ int xppc_c_flasher_bunch(int32_t str, int32_t dom, ppc_hip_bunch* out);
 double xppc_c_particle_bunch(float x, float y, float z, float t, float E, float dr, int32_t type, float nx,
                              float ny, float nz, ppc_hip_bunch* out);
 
+/* the bunches of the IceTray entry points with direction n (gpu/f2k.cxx:507-528): addp_clst's
+ * segment of length dr of a particle of velocity beta, and addp_mopo's segment with light
+ * yield `light`, luminescence time tau and bare-track fraction fr (returns the Poisson mean of
+ * its photon count); num is left at 0 */
+int xppc_c_clst_bunch(float x, float y, float z, float t, float nx, float ny, float nz, float dr, float beta,
+                      ppc_hip_bunch* out);
+double xppc_c_mopo_bunch(float x, float y, float z, float t, float nx, float ny, float nz, float light, float length,
+                         float beta, float tau, float fr, ppc_hip_bunch* out);
+
 /* photons flone(num) propagates for a flasher pulse: num*eff/dppm/xR^2 (gpu/f2k.cxx:586-588) */
 uint64_t xppc_c_flasher_photons(uint64_t num);
 

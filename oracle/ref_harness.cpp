@@ -135,6 +135,26 @@ double ref_particle(float x, float y, float zz, float t, float E, float dr, int 
   return mean;
 }
 
+// The bunches the IceTray entry points addp_clst() and addp_mopo() build (gpu/f2k.cxx:507-528)
+// for direction n (set as sett() does), taken from the reference's own global bunch after
+// the call; the photons they queue are dropped.  ref_mopo returns the Poisson mean of its count.
+void ref_clst(float x, float y, float zz, float t, float nx, float ny, float nz, float dr, float beta,
+              orc_bunch* out) {
+  p.n.x = nx, p.n.y = ny, p.n.z = nz;
+  addp_clst(x, y, zz, t, 1, dr, beta);
+  from_photon(p, *out);
+  pn = 0, pk = 0;
+}
+
+double ref_mopo(float x, float y, float zz, float t, float nx, float ny, float nz, float light, float length,
+                float beta, float tau, float fr, orc_bunch* out) {
+  p.n.x = nx, p.n.y = ny, p.n.z = nz;
+  addp_mopo(x, y, zz, t, light, length, beta, tau, fr);
+  from_photon(p, *out);
+  pn = 0, pk = 0;
+  return light * q.eff / ovr;
+}
+
 // the flasher bunch flset() builds (gpu/f2k.cxx:630-759)
 void ref_flasher(int str, int dom, orc_bunch* out) {
   flset(str, dom);

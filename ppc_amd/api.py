@@ -91,6 +91,7 @@ def hip():
         lib.ppc_hip_capacity.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
         lib.ppc_hip_upload.argtypes = [c_p, c_p]
         lib.ppc_hip_seed.argtypes = [c_p, ctypes.c_uint32]
+        lib.ppc_hip_seed_photons.argtypes = [c_p, ctypes.c_uint64]
         lib.ppc_hip_get_states.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]
         lib.ppc_hip_set_states.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32]
         lib.ppc_hip_launch.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32]
@@ -124,6 +125,9 @@ def xppc():
         lib.xppc_c_particle_bunch.restype = ctypes.c_double
         lib.xppc_c_particle_bunch.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int32] + [ctypes.c_float] * 3 + [
             ctypes.POINTER(Bunch)]
+        lib.xppc_c_clst_bunch.argtypes = [ctypes.c_float] * 9 + [ctypes.POINTER(Bunch)]
+        lib.xppc_c_mopo_bunch.restype = ctypes.c_double
+        lib.xppc_c_mopo_bunch.argtypes = [ctypes.c_float] * 12 + [ctypes.POINTER(Bunch)]
         lib.xppc_c_flasher_photons.restype = ctypes.c_uint64
         lib.xppc_c_flasher_photons.argtypes = [ctypes.c_uint64]
         lib.xppc_c_zshift.restype = ctypes.c_float
@@ -200,6 +204,22 @@ class Detector:
             raise PPCError("flasher configuration failed")
         b.num, b.q = num, seg
         return b
+
+    def clst_bunch(self, x, y, z, t, n, dr, beta, num=0, seg=0):
+        """addp_clst's bunch: a segment of length dr of a particle of velocity beta"""
+        b = Bunch()
+        if self.lib.xppc_c_clst_bunch(x, y, z, t, n[0], n[1], n[2], dr, beta, ctypes.byref(b)) != 0:
+            raise PPCError("clst bunch failed")
+        b.num, b.q = num, seg
+        return b
+
+    def mopo_bunch(self, x, y, z, t, n, light, length, beta, tau, fr, num=None, seg=0):
+        """addp_mopo's bunch and the Poisson mean of its photon count"""
+        b = Bunch()
+        mean = self.lib.xppc_c_mopo_bunch(x, y, z, t, n[0], n[1], n[2], light, length, beta, tau, fr, ctypes.byref(b))
+        b.num = int(round(mean)) if num is None else num
+        b.q = seg
+        return b, mean
 
     def particle_bunch(self, x, y, z, t, E, dr, ptype, n, num=None, seg=0):
         """bunch of an f2k particle (type 0 muon, 11-14 EM, 101-107 hadron); num
@@ -301,6 +321,11 @@ class Device:
 
     def seed(self, sv):
         _check(self.lib.ppc_hip_seed(self.ctx, sv))
+
+    def seed_photons(self, seed):
+        """seed != 0: every photon of later launches starts from a state of its own hashed
+        from (seed, photon index) (common random numbers across launches); 0: off"""
+        _check(self.lib.ppc_hip_seed_photons(self.ctx, seed))
 
     def states(self):
         a = np.zeros(self.lanes, dtype=np.uint64)

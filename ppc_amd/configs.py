@@ -416,7 +416,47 @@ def write_c2t9(path, nrnd=131072):
     return write_c2(path, nrnd=nrnd, ntilt=9)
 
 
-_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9, "cbig": write_cbig}
+def _fibonacci_dirs(n):
+    """n nearly uniform unit directions (Fibonacci lattice on the sphere)"""
+    k = np.arange(n) + 0.5
+    ct = 1 - 2 * k / n
+    st = np.sqrt(1 - ct * ct)
+    ph = math.pi * (1 + math.sqrt(5.0)) * k
+    return np.column_stack([st * np.cos(ph), st * np.sin(ph), ct])
+
+
+def write_c1ng(path, nrnd=131072):
+    """The 7-string ice with nextgen sensors (multi-PMT modules, ini.cxx:731-761): om.conf
+    with a 12-PMT module (rings of 4 PMTs at zenith 30, 90 and 150 deg; an ellipsoid of
+    radii 0.16/0.205 m) and a 2-PMT module (up and down; 0.15/0.27 m), om.map placing them on
+    strings 1 and 2 (the other strings keep the one-PMT DOM), their wavelength acceptances
+    om.wv_<type>.0, and the direction mesh om.dirs of the PMT-to-module area ratio."""
+    write_c1(path, nrnd=nrnd)
+    with open(os.path.join(path, "om.conf"), "w") as f:
+        pmts = [(30.0, 90.0 * k) for k in range(4)] + [(90.0, 45.0 + 90.0 * k) for k in range(4)] + \
+               [(150.0, 90.0 * k) for k in range(4)]
+        f.write("mdom 1 0.35 0.50 0.16 0.205 12 %.1f %.1f\n" % pmts[0])
+        for th, ph in pmts[1:]:
+            f.write("%.1f %.1f\n" % (th, ph))
+        f.write("degg 2 0.9 0.25 0.15 0.27 2 0.0 0.0\n180.0 0.0\n")
+    with open(os.path.join(path, "om.dirs"), "w") as f:
+        for k, d in enumerate(_fibonacci_dirs(2000)):
+            f.write("%d %.7f %.7f %.7f\n" % (k, d[0], d[1], d[2]))
+    with open(os.path.join(path, "om.map"), "w") as f:
+        for s, m in ((1, 1), (2, 2)):
+            for d in range(1, 61):
+                f.write("%d %d %d\n" % (s, d, m))
+    # acceptance per nm on the scale of the one-PMT DOM's (dppm * doma * omav, gpu/ini.cxx:1425-1433)
+    for m, (lo, hi, amp) in ((1, (290.0, 620.0, 60.0)), (2, (300.0, 600.0, 40.0))):
+        with open(os.path.join(path, "om.wv_%d.0" % m), "w") as f:
+            for w in np.arange(lo, hi + 1e-3, 10.0):
+                x = (w - lo) / (hi - lo)
+                f.write("%.1f %.6f\n" % (w, amp * math.sin(math.pi * x) ** 2))
+    return path
+
+
+_WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9, "cbig": write_cbig,
+            "c1ng": write_c1ng}
 
 
 def ensure(name, nrnd=131072):

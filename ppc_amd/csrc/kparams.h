@@ -53,6 +53,7 @@ struct KParams {
   int mnum0, mnum1, mcut0, mcut1;
   int nsc, gsize;
   int uniform_dom;  // every DOM has R = OMR and F = 1
+  int scat_frame;   // some layer has k1 != 1 or k2 != 1: scatter in the anisotropic frame
   // scattering-function constants: 1/(1-sf), 1-g^2, (1-g)^2, 1/(2g), 1/sf; [0] bulk ice, [1] hole ice
   float hg_isf1[2], hg_a[2], hg_b[2], hg_i2g[2], hg_isf[2];
 
@@ -77,7 +78,6 @@ struct KParams {
   // ---- RNG ----
   const uint32_t* rm;  // multiplier per lane
   uint2* rs;           // (x, c) per lane
-  uint4* lane_info;    // per lane: photon index, segment, MWC state (x, c) it started from
 
   // ---- launch ----
   const ppc_hip_bunch* pz;
@@ -91,6 +91,7 @@ struct KParams {
   uint32_t rec_stride, rec_max;
   int refill, deterministic;
   uint32_t chunk;  // photons claimed per atomic on the global counter
+  uint64_t photon_seed;  // != 0: each photon starts from a state hashed from (photon_seed, index)
   const uint16_t* bvar;  // ice variant of each bunch (nullptr: variant 0)
   ppc_hip_adj_record* hit_photons;  // start state of every photon that hits, or nullptr
   uint32_t hit_photons_cap;

@@ -59,6 +59,16 @@ hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, sizeof(T) * (n > 0 ? n : 1));
 }
 
+__global__ void zero_words(uint32_t* p, uint32_t n) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_words, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 struct ppc_hip_ctx {
@@ -101,7 +111,6 @@ struct ppc_hip_ctx {
   uint16_t* d_bvar = nullptr;     // per-bunch ice variant of a batch launch
   uint16_t* h_bvar = nullptr;
   float* d_clear = nullptr;       // clearance map of the string cylinders
-  uint4* d_info = nullptr;        // per lane: photon, segment, start state
   // hit post-processing tables and output (ppc_hip_upload_accept / ppc_hip_accept)
   ppc_hip_sensor* d_sensors = nullptr;
   float* d_pmt_dirs = nullptr;
@@ -122,6 +131,7 @@ struct ppc_hip_ctx {
   size_t adj_wcap = 0;
   float* d_adj_dp = nullptr;
   double* d_adj_grad = nullptr;
+  uint64_t photon_seed = 0;       // ppc_hip_seed_photons
   hipGraph_t graph = nullptr;  // ppc_hip_graph_capture: the graph and the buffers it reads
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_num = 0;
@@ -149,7 +159,8 @@ int set_layers(ppc_hip_ctx* c, const ppc_hip_layer* lay, int nvar) {
   img.resize(img.size() + sizeof(ppc_hip_layer) * P.size * nvar);
   memcpy(img.data() + P.off_lay, lay, sizeof(ppc_hip_layer) * P.size * nvar);
   img.resize((img.size() + 15) & ~size_t(15));
-  if (img.size() + sizeof(float) * 2 * P.size > 64 * 1024)
+  // the image, a 16-byte photon record per lane and the adjoint's block sums share the LDS
+  if (img.size() + 16 * (size_t)c->cfg.threads + sizeof(float) * 2 * P.size > 64 * 1024)
     return fail(PPC_HIP_E_CAPACITY, "LDS image of %zu bytes too large (%d ice variants)", img.size(), nvar);
   HIPCHK(hipStreamSynchronize(c->stream));  // a launch in flight may still read the old image
   if (c->d_image) (void)hipFree(c->d_image), c->d_image = nullptr;
@@ -244,7 +255,6 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   chk(dalloc(&c->d_bvar, k.max_bunches));
   chk(hipHostMalloc((void**)&c->h_bvar, sizeof(uint16_t) * k.max_bunches, hipHostMallocDefault));
   if (k.record_starts) chk(dalloc(&c->d_hphot, k.max_hits));
-  chk(dalloc(&c->d_info, c->lanes));
   chk(dalloc(&c->d_ctr_adj, 1));
   chk(hipHostMalloc((void**)&c->h_ctr_adj, sizeof(ppc::Counters), hipHostMallocDefault));
   if (e != hipSuccess) {
@@ -262,7 +272,7 @@ int ppc_hip_close(ppc_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->d_pz, (void*)c->d_pstart, (void*)c->d_hits, (void*)c->d_hit_photon, (void*)c->d_ctr,
                   (void*)c->d_rec, (void*)c->d_rs, (void*)c->d_rm, (void*)c->d_image, (void*)c->d_lp, (void*)c->d_wv,
-                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj, (void*)c->d_clear, (void*)c->d_info, (void*)c->d_sensors,
+                  (void*)c->d_dom, (void*)c->d_domF, (void*)c->d_P, (void*)c->d_bvar, (void*)c->d_hphot, (void*)c->d_ctr_adj, (void*)c->d_clear, (void*)c->d_sensors,
                   (void*)c->d_pmt_dirs, (void*)c->d_rat, (void*)c->d_adoms, (void*)c->d_awvs, (void*)c->d_accepted,
                   (void*)c->d_adj_rec, (void*)c->d_adj_w, (void*)c->d_adj_dp, (void*)c->d_adj_grad})
     if (p) (void)hipFree(p);
@@ -405,7 +415,13 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
     if (bytes) memcpy(img.data() + off, src, bytes);
     return (int)off;
   };
-  {  // per layer (k1, k2, ra, rb), then (1/k1, 1/k2, 1/(k1 k2), k1 k2) for the scattering frame
+  P.scat_frame = 0;
+  for (int i = 0; i < t->size; i++)
+    if (t->az[i].k1 != 1.f || t->az[i].k2 != 1.f) P.scat_frame = 1;
+  // the per-layer anisotropy table is read by the scattering frame, birefringence and the old
+  // absorption anisotropy only: without them it stays out of the LDS image
+  if (P.scat_frame || t->sum > 0 || t->hr > 0 || t->fr != 0) {
+    // per layer (k1, k2, ra, rb), then (1/k1, 1/k2, 1/(k1 k2), k1 k2) for the scattering frame
     std::vector<float4> az(2 * t->size);
     for (int i = 0; i < t->size; i++) {
       const ppc_hip_aniz& a = t->az[i];
@@ -512,7 +528,6 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.domF = c->d_domF;
   P.rm = c->d_rm;
   P.rs = c->d_rs;
-  P.lane_info = c->d_info;
   P.pz = c->d_pz;
   P.pstart = c->d_pstart;
   P.hits = c->d_hits;
@@ -535,6 +550,12 @@ int ppc_hip_seed(ppc_hip_ctx* c, uint32_t sv) {
   if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
   std::vector<uint64_t> st(c->lanes, (uint64_t)(uint32_t)(362436069u + sv) | ((uint64_t)1234567u << 32));
   return ppc_hip_set_states(c, st.data(), c->lanes);
+}
+
+int ppc_hip_seed_photons(ppc_hip_ctx* c, uint64_t seed) {
+  if (!c) return fail(PPC_HIP_E_ARG, "NULL context");
+  c->photon_seed = seed;
+  return PPC_HIP_OK;
 }
 
 int ppc_hip_set_states(ppc_hip_ctx* c, const uint64_t* states, int32_t lanes) {
@@ -584,6 +605,7 @@ int stage(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, const uint16_t* variant,
   if (nb && variant) memcpy(c->h_bvar, variant, sizeof(uint16_t) * nb);
   *c->h_P = c->P;
   c->h_P->nb = nb, c->h_P->num = num;
+  c->h_P->photon_seed = c->photon_seed;
   c->h_P->bvar = variant ? c->d_bvar : nullptr;
   {  // claim chunks of photons per wave: large enough to keep the global counter cold, small
      // enough that the last chunks of a launch still spread over all waves: each wave
@@ -611,6 +633,81 @@ int enqueue(ppc_hip_ctx* c, uint32_t nb, bool variants, bool timed) {
     HIPCHK(ppc::launch_propagate(*c->h_P, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream,
                                  c->cfg.count_events != 0, false));
   if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+  return PPC_HIP_OK;
+}
+
+
+// device buffers of one adjoint call: replay records given by the caller, LLH weights,
+// d(p)/d(a) then d(s1)/d(b), the FP64 gradient, the launch counters
+struct AdjBuffers {
+  ppc_hip_adj_record* rec;
+  float* weight;
+  float* dp;
+  double* grad;
+  ppc::Counters* ctr;
+};
+
+int adjoint_check(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                  const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                  const float* dpda, const float* ds1db, double* grad) {
+  if (!c || !grad || !dpda || !ds1db || (nrec && !rec) || (nseg && !weight)) return fail(PPC_HIP_E_ARG, "NULL argument");
+  if (!c->uploaded) return fail(PPC_HIP_E_STATE, "adjoint before upload");
+  if (c->P.hr > 0) return fail(PPC_HIP_E_ARG, "the adjoint does not model hole ice");
+  if (!rec) {  // the hitting photons the last launch recorded
+    if (!c->d_hphot) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
+    if (!c->fwd_waited) return fail(PPC_HIP_E_STATE, "wait for the forward launch before its adjoint");
+    if (num != c->last_num) return fail(PPC_HIP_E_ARG, "adjoint of %u photons, the launch had %u", num, c->last_num);
+  } else {
+    for (uint32_t k = 0; k < nrec; k++)
+      if (rec[k].lane >= (uint32_t)c->lanes || rec[k].photon >= num)
+        return fail(PPC_HIP_E_ARG, "adjoint record %u: photon %u lane %u out of range", k, rec[k].photon,
+                    rec[k].lane);
+  }
+  (void)bunches, (void)nb;
+  return PPC_HIP_OK;
+}
+
+// the replay launch of an adjoint into the buffers B (checked by adjoint_check)
+int adjoint_run(ppc_hip_ctx* c, const AdjBuffers& B, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* st) {
+  if (!rec) nrec = c->fwd_nphot;
+  int rc = stage(c, bunches, nullptr, nb, num);
+  if (rc != PPC_HIP_OK) return rc;
+  const int size = c->P.size;
+  // the stream is idle here (stage waited for the staging copy; synchronous calls below)
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rec && nrec) HIPCHK(hipMemcpy(B.rec, rec, sizeof(ppc_hip_adj_record) * nrec, hipMemcpyHostToDevice));
+  if (nseg) HIPCHK(hipMemcpy(B.weight, weight, sizeof(float) * nseg * c->P.gsize, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(B.dp, dpda, sizeof(float) * size, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(B.dp + size, ds1db, sizeof(float) * size, hipMemcpyHostToDevice));
+  // the gradient starts at zero: cleared by a kernel (no memset node in a captured stream)
+  HIPCHK(launch_zero(reinterpret_cast<uint32_t*>(B.grad), (uint32_t)(sizeof(double) * 2 * size / 4), c->stream));
+  KParams& H = *c->h_P;
+  H.adj_rec = rec ? B.rec : c->d_hphot, H.adj_nrec = nrec, H.adj_nseg = nseg;
+  H.adj_weight = B.weight, H.adj_dpda = B.dp, H.adj_ds1db = B.dp + size;
+  H.adj_grad = B.grad;
+  H.ctr = B.ctr;            // the forward launch's counters and hits stay as they are
+  H.hit_photons = nullptr;
+  HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(ppc::launch_reset(B.ctr, c->stream));
+  HIPCHK(hipEventRecord(c->ev_a0, c->stream));
+  if (nrec)
+    HIPCHK(ppc::launch_propagate(H, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, false, true));
+  HIPCHK(hipEventRecord(c->ev_a1, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_ctr_adj, B.ctr, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(grad, B.grad, sizeof(double) * 2 * size, hipMemcpyDeviceToHost));
+  if (st) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1));
+    const ppc::Counters& k = *c->h_ctr_adj;
+    *st = ppc_hip_stats{};
+    st->photons = k.photons, st->steps = k.steps, st->killed = k.killed, st->nonfinite = k.nonfinite;
+    st->device_ms = ms, st->lanes = c->lanes;
+  }
   return PPC_HIP_OK;
 }
 
@@ -770,24 +867,12 @@ int ppc_hip_hit_photons(ppc_hip_ctx* c, ppc_hip_adj_record* rec, uint32_t cap, u
 int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
                     const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
                     const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* st) {
-  if (!c || !grad || !dpda || !ds1db || (nrec && !rec) || (nseg && !weight)) return fail(PPC_HIP_E_ARG, "NULL argument");
-  if (!c->uploaded) return fail(PPC_HIP_E_STATE, "adjoint before upload");
-  if (c->P.hr > 0) return fail(PPC_HIP_E_ARG, "the adjoint does not model hole ice");
-  if (!rec) {  // the hitting photons the last launch recorded
-    if (!c->d_hphot) return fail(PPC_HIP_E_STATE, "context opened without record_starts");
-    if (!c->fwd_waited) return fail(PPC_HIP_E_STATE, "wait for the forward launch before its adjoint");
-    if (num != c->last_num) return fail(PPC_HIP_E_ARG, "adjoint of %u photons, the launch had %u", num, c->last_num);
-    nrec = c->fwd_nphot;
-  } else {
-    for (uint32_t k = 0; k < nrec; k++)
-      if (rec[k].lane >= (uint32_t)c->lanes || rec[k].photon >= num)
-        return fail(PPC_HIP_E_ARG, "adjoint record %u: photon %u lane %u out of range", k, rec[k].photon,
-                    rec[k].lane);
-  }
-  int rc = stage(c, bunches, nullptr, nb, num);
+  int rc = adjoint_check(c, bunches, nb, num, rec, nrec, weight, nseg, dpda, ds1db, grad);
   if (rc != PPC_HIP_OK) return rc;
-  const int size = c->P.size;
+  HIPCHK(hipSetDevice(c->device));
+  // the adjoint's buffers grow on demand and stay with the context
   if (rec && nrec > c->adj_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_adj_rec) (void)hipFree(c->d_adj_rec);
     c->d_adj_rec = nullptr;
     HIPCHK(dalloc(&c->d_adj_rec, nrec));
@@ -795,6 +880,7 @@ int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, u
   }
   const size_t wn = (size_t)std::max<uint32_t>(nseg, 1) * c->P.gsize;
   if (wn > c->adj_wcap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_adj_w) (void)hipFree(c->d_adj_w);
     c->d_adj_w = nullptr;
     HIPCHK(dalloc(&c->d_adj_w, wn));
@@ -802,39 +888,8 @@ int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, u
   }
   if (!c->d_adj_dp) HIPCHK(dalloc(&c->d_adj_dp, 2 * PPC_HIP_MAXLYS));
   if (!c->d_adj_grad) HIPCHK(dalloc(&c->d_adj_grad, 2 * PPC_HIP_MAXLYS));
-  // the stream is idle here (stage waited for the staging copy; synchronous calls below)
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (rec && nrec) HIPCHK(hipMemcpy(c->d_adj_rec, rec, sizeof(ppc_hip_adj_record) * nrec, hipMemcpyHostToDevice));
-  if (nseg) HIPCHK(hipMemcpy(c->d_adj_w, weight, sizeof(float) * nseg * c->P.gsize, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_adj_dp, dpda, sizeof(float) * size, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_adj_dp + size, ds1db, sizeof(float) * size, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(c->d_adj_grad, 0, sizeof(double) * 2 * size));
-  KParams& H = *c->h_P;
-  H.adj_rec = rec ? c->d_adj_rec : c->d_hphot, H.adj_nrec = nrec, H.adj_nseg = nseg;
-  H.adj_weight = c->d_adj_w, H.adj_dpda = c->d_adj_dp, H.adj_ds1db = c->d_adj_dp + size;
-  H.adj_grad = c->d_adj_grad;
-  H.ctr = c->d_ctr_adj;     // the forward launch's counters and hits stay as they are
-  H.hit_photons = nullptr;
-  HIPCHK(hipMemcpyAsync(c->d_pz, c->h_pz, sizeof(ppc_hip_bunch) * nb, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_pstart, c->h_pstart, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_P, c->h_P, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(ppc::launch_reset(c->d_ctr_adj, c->stream));
-  HIPCHK(hipEventRecord(c->ev_a0, c->stream));
-  if (nrec)
-    HIPCHK(ppc::launch_propagate(H, c->d_P, c->cfg.blocks, c->cfg.threads, c->lds_bytes, c->stream, false, true));
-  HIPCHK(hipEventRecord(c->ev_a1, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_ctr_adj, c->d_ctr_adj, sizeof(ppc::Counters), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(grad, c->d_adj_grad, sizeof(double) * 2 * size, hipMemcpyDeviceToHost));
-  if (st) {
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1));
-    const ppc::Counters& k = *c->h_ctr_adj;
-    *st = ppc_hip_stats{};
-    st->photons = k.photons, st->steps = k.steps, st->killed = k.killed, st->nonfinite = k.nonfinite;
-    st->device_ms = ms, st->lanes = c->lanes;
-  }
-  return PPC_HIP_OK;
+  AdjBuffers B{c->d_adj_rec, c->d_adj_w, c->d_adj_dp, c->d_adj_grad, c->d_ctr_adj};
+  return adjoint_run(c, B, bunches, nb, num, rec, nrec, weight, nseg, dpda, ds1db, grad, st);
 }
 
 int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {

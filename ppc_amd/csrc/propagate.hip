@@ -48,6 +48,14 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 }
 
 // ---------------------------------------------------------------- RNG
+// SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014): the start state of photon i
+// when photons are seeded one by one
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 // Multiply-with-carry: t = a*x + c (one v_mad_u64_u32), x = lo(t), c = hi(t);
 // uniform in (0,1] from the top 23 bits of x (gpu/pro.cu:47-57).
 struct Mwc {
@@ -60,17 +68,20 @@ struct Mwc {
   }
 };
 
-// gamma(k) variate (gpu/pro.cu:59-85): Weibull rejection for k<1, Cheng otherwise
+// gamma(k) variate (gpu/pro.cu:59-85): Weibull rejection for k<1, Cheng otherwise.  k = 0
+// (a particle bunch without longitudinal profile) keeps the reference's powf values
+// pow(0, 0) = 1 and pow(z, inf) = 0, inf, 1 for z <, >, = 1, where exp2(y log2 x) would give
+// NaN: the loop then draws until z < 1 <= z + e and returns 0, as the reference does.
 __device__ float gamma_rnd(float k, Mwc& s) {
   float x;
   if (k < 1) {
     float c = frcp(k);
-    float d = (1 - k) * fpow(k, frcp(c - 1));
+    float d = (1 - k) * (k > 0 ? fpow(k, frcp(c - 1)) : 1.f);
     float z, e;
     do {
       z = -flog(s());
       e = -flog(s());
-      x = fpow(z, c);
+      x = k > 0 ? fpow(z, c) : z < 1 ? 0.f : z > 1 ? __builtin_huge_valf() : 1.f;
     } while (z + e < d + x);
   } else {
     float b = k - 1.38629436112f;  // log 4
@@ -125,23 +136,22 @@ __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
     t = r0; r0 = r2; r2 = t;
     ax = 2;
   }
-  float a = frsq(r0 + r1);
-  float p1x = -ny * a, p1y = nx * a, p1z = 0;
-  float b = frsq(r0 + r2);
-  float p2x = -nz * b, p2y = 0, p2z = nx * b;
-  float qx = p1x - p2x, qy = p1y - p2y, qz = p1z - p2z;
-  p2x += p1x, p2y += p1y, p2z += p1z;
-  // |p1| = |p2| = 1 and p1.p2 = ny nz a b, so |p1 - p2|^2 = 2 - cc and |p1 + p2|^2 = 2 + cc
+  // the perpendicular pair p1 = a (-ny, nx, 0), p2 = b (-nz, 0, nx) (unit: a, b normalise),
+  // turned by 45 degrees into q = (p1 - p2) / |p1 - p2| and p = (p1 + p2) / |p1 + p2|; with
+  // p1.p2 = ny nz a b = cc / 2 the norms are sqrt(2 - cc) and sqrt(2 + cc).  The new direction
+  // n (1 - cs) + si (cos u q + sin u p) is summed component by component from the factors
+  // c1 = si cos u / |p1 - p2| and c2 = si sin u / |p1 + p2|: with E = a (c1 + c2) and
+  // F = b (c1 - c2) the turn adds (-ny E + nz F, nx E, -nx F)
+  const float a = frsq(r0 + r1), b = frsq(r0 + r2);
   const float cc = 2 * ny * nz * a * b;
-  float w = frsq(2 - cc);
-  p1x = qx * w, p1y = qy * w, p1z = qz * w;
-  w = frsq(2 + cc);
-  p2x *= w, p2y *= w, p2z *= w;
-  float u = s();
-  float ps = __builtin_amdgcn_sinf(u), pc = __builtin_amdgcn_cosf(u);
-  nx -= cs * nx - si * (pc * p1x + ps * p2x);
-  ny -= cs * ny - si * (pc * p1y + ps * p2y);
-  nz -= cs * nz - si * (pc * p1z + ps * p2z);
+  const float u = s();
+  const float c1 = si * __builtin_amdgcn_cosf(u) * frsq(2 - cc);
+  const float c2 = si * __builtin_amdgcn_sinf(u) * frsq(2 + cc);
+  const float E = a * (c1 + c2), F = b * (c1 - c2), omc = 1 - cs;
+  const float mx = nx * omc + (nz * F - ny * E);
+  const float my = ny * omc + nx * E;
+  const float mz = nz * omc - nx * F;
+  nx = mx, ny = my, nz = mz;
   if (NORM) normalize3(nx, ny, nz);
   if (ax == 1) {
     float t = nx; nx = ny; ny = t;
@@ -464,7 +474,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   const KParams& P = *Pp;
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
-  float* gacc = reinterpret_cast<float*>(lds + P.image_words);  // ADJ: [2][size] block sums
+  // per lane, after the image: the photon's index, its segment and the MWC state it started
+  // from (written at emission, read back only for hits, records and the adjoint's second pass)
+#define info (reinterpret_cast<uint32_t*>(lds + P.image_words) + threadIdx.x)
+  // (four words per lane, a stride of blockDim.x apart: photon, segment, x, c)
+  float* gacc = reinterpret_cast<float*>(lds + P.image_words + blockDim.x);  // ADJ: [2][size] block sums
   if (ADJ)
     for (int k = threadIdx.x; k < 2 * P.size; k += blockDim.x) gacc[k] = 0.f;
   __syncthreads();
@@ -504,9 +518,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   // photons of the lane (rare), event counts of the lane (COUNT builds only)
   uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;
-  // per lane in memory: the photon's index, its segment and the MWC state it started from
-  // (read back only for hits, records and the adjoint's second pass)
-  // (indexed by gid at each use rather than held as a per-lane 64-bit pointer)
   // ADJ: each record's photon runs twice from its start state: pass 0 sums the LLH
   // weights of its hits (Gsum), pass 1 scores the path with G, the weight of the hits
   // still to come
@@ -525,9 +536,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         uint32_t i = 0xffffffffu;
         if (ADJ) {  // one record after the other per lane
           if (!live && pend) {  // the scoring pass of the photon just summed
-            const uint4 in = P.lane_info[gid];
-            s.x = in.z, s.c = in.w;
-            i = in.x;
+            s.x = info[2 * blockDim.x], s.c = info[3 * blockDim.x];
+            i = info[0];
             G = Gsum, pass1 = true, pend = false;  // nw carries over from pass 0
           } else if (!live && next_static < nwork) {
             // (an exhausted lane's counter stays put: advancing it at every refill would
@@ -570,6 +580,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         const bool start = !live && i < P.num;
         n_photons += __popcll(__ballot(start));
         if (start) {
+          if (!ADJ && P.photon_seed != 0) {
+            // a start state of the photon's own, a hash of (seed, i) on its lane's multiplier:
+            // launches with the same seed draw the same numbers for photon i whatever the
+            // other photons did (common random numbers); 0 < c < a, x odd
+            const uint64_t h = mix64(P.photon_seed + (uint64_t)(i + 1u) * 0x9E3779B97F4A7C15ull);
+            s.x = (uint32_t)h | 1u;
+            s.c = 1u + __umulhi((uint32_t)(h >> 32), s.a - 2u);
+          }
           const uint32_t x0 = s.x, c0 = s.c;
           if (!ADJ && P.rec_stride != 0 && i % P.rec_stride == 0 && i / P.rec_stride < P.rec_max) {
             ppc_hip_record rc;
@@ -579,7 +597,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           ph.clear = 0.f;
           uint32_t seg;
           int fla = emit(P, i, s, ph, seg);
-          P.lane_info[gid] = make_uint4(i, seg, x0, c0);
+          info[0] = i, info[blockDim.x] = seg, info[2 * blockDim.x] = x0, info[3 * blockDim.x] = c0;
           ph.TOT = -flog(s());
           ph.SCA = 0;
           ph.old = fla;
@@ -847,7 +865,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       if (om == ofla)
         TOT = 0;
       else {
-        const uint32_t seg = P.lane_info[gid].y;
+        const uint32_t seg = info[blockDim.x];
         if (seg < P.adj_nseg) {
           const float wh = P.adj_weight[(size_t)seg * P.gsize + om];
           if (!pass1)
@@ -865,8 +883,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         TOT = 0;
       else {
         ppc_hip_hit h;
-        const uint4 in = P.lane_info[gid];
-        h.i = om, h.t = r.w, h.n = in.y, h.z = ph.nhw >> 16;
+        h.i = om, h.t = r.w, h.n = info[blockDim.x], h.z = ph.nhw >> 16;
         h.pth = acosf(n.z);
         h.pph = atan2f(n.y, n.x);
         sqd *= P.xR - 1;
@@ -888,7 +905,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         const uint32_t slot = first + lanes_below(hm);
         if (slot < P.hnum) {
           P.hits[slot] = h;
-          P.hit_photon[slot] = in.x;
+          P.hit_photon[slot] = info[0];
         }
         ph.nhw++;
         if (P.xR == 1)
@@ -913,8 +930,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         const uint32_t slot = first + lanes_below(em);
         if (slot < P.hit_photons_cap) {
           ppc_hip_adj_record rc;
-          const uint4 in = P.lane_info[gid];
-          rc.photon = in.x, rc.x0 = in.z, rc.c0 = in.w, rc.lane = gid;
+          rc.photon = info[0], rc.x0 = info[2 * blockDim.x], rc.c0 = info[3 * blockDim.x], rc.lane = gid;
           P.hit_photons[slot] = rc;
         }
       }
@@ -1007,29 +1023,28 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         xi = fminf(fmaxf(xi, 0.f), 2.f);
         if (ADJ && G != 0.f && !(EXT && hole))
           atomicAdd(&gacc[P.size + J], G * w.la * P.adj_ds1db[J] * frcp(w.sca(T, J)));
-        // (k1, k2, k1 k2) of the layer and their reciprocals, prepared on the host
-        const float4 a = T.az[J], ar = T.azr[J];
-        const bool frame = !(EXT && hole);
-        if (frame) {  // into the frame of anisotropic scattering
+        // (k1, k2, k1 k2) of the layer and their reciprocals, prepared on the host.  Where
+        // every layer has k1 = k2 = 1 the frame is the identity: the way in and back (two
+        // linear maps and two normalisations) would only round, and is left out
+        if (P.scat_frame && !(EXT && hole)) {
+          const float4 a = T.az[J], ar = T.azr[J];
+          // into the frame of anisotropic scattering
           float n1 = (P.azx * n.x + P.azy * n.y) * a.x;
           float n2 = (-P.azy * n.x + P.azx * n.y) * a.y;
           n.x = n1 * P.azx - n2 * P.azy;
           n.y = n1 * P.azy + n2 * P.azx;
           n.z *= ar.z;
           normalize3(n.x, n.y, n.z);
-        }
-        if (frame)  // the way back ends in a normalisation
-          rotate<false>(xi, fsqrt(xi * (2 - xi)), n, s);
-        else
-          rotate(xi, fsqrt(xi * (2 - xi)), n, s);
-        if (frame) {  // and back
-          float n1 = (P.azx * n.x + P.azy * n.y) * ar.x;
-          float n2 = (-P.azy * n.x + P.azx * n.y) * ar.y;
+          rotate<false>(xi, fsqrt(xi * (2 - xi)), n, s);  // the way back ends in a normalisation
+          // and back
+          n1 = (P.azx * n.x + P.azy * n.y) * ar.x;
+          n2 = (-P.azy * n.x + P.azx * n.y) * ar.y;
           n.x = n1 * P.azx - n2 * P.azy;
           n.y = n1 * P.azy + n2 * P.azx;
           n.z *= ar.w;
           normalize3(n.x, n.y, n.z);
-        }
+        } else
+          rotate(xi, fsqrt(xi * (2 - xi)), n, s);
       }
     }
     ph.r = r, ph.n = n;
@@ -1038,6 +1053,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     }
   }
 
+#undef info
   if (ADJ) {  // block sums into the FP64 gradient
     __syncthreads();
     for (int k = threadIdx.x; k < 2 * P.size; k += blockDim.x)
@@ -1094,6 +1110,7 @@ hipError_t launch_propagate(const KParams& P, const KParams* dP, int blocks, int
   const bool ext = P.sum > 0 || P.hr > 0 || P.fr != 0;
   kfun f = adjoint ? pick<false, true>(P.tmod, ext)
                    : count ? pick<true, false>(P.tmod, ext) : pick<false, false>(P.tmod, ext);
+  lds_bytes += sizeof(uint4) * threads;  // per-lane photon records
   if (adjoint) lds_bytes += sizeof(float) * 2 * P.size;
   hipLaunchKernelGGL(f, dim3(blocks), dim3(threads), lds_bytes, stream, dP);
   return hipGetLastError();

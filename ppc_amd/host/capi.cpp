@@ -121,6 +121,22 @@ double xppc_c_particle_bunch(float x, float y, float z, float t, float E, float 
   return mean;
 }
 
+int xppc_c_clst_bunch(float x, float y, float z, float t, float nx, float ny, float nz, float dr, float beta,
+                      ppc_hip_bunch* out) {
+  if (!g_init || !out) return -1;
+  *out = clst_bunch(x, y, z, t, nx, ny, nz, dr, beta);
+  out->num = 0;
+  return 0;
+}
+
+double xppc_c_mopo_bunch(float x, float y, float z, float t, float nx, float ny, float nz, float light, float length,
+                         float beta, float tau, float fr, ppc_hip_bunch* out) {
+  if (!g_init || !out) return -1;
+  double mean = mopo_bunch(x, y, z, t, nx, ny, nz, light, length, beta, tau, fr, *out);
+  out->num = 0;
+  return mean;
+}
+
 uint64_t xppc_c_flasher_photons(uint64_t num) {
   return (unsigned long long)(num * (det.eff / dppm) / det.ovr);
 }

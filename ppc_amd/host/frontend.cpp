@@ -533,23 +533,34 @@ void addp(float rx, float ry, float rz, float t, float E, float dr, int type, fl
   addh(poidev(particle_setup(rx, ry, rz, t, E, dr, type, scale)));
 }
 
-void addp_clst(float rx, float ry, float rz, float t, unsigned long long n, float dr, float beta) {  // 507-518
+// set p for a segment of n photons of a particle of velocity beta (gpu/f2k.cxx:507-518)
+static void clst_setup(float rx, float ry, float rz, float t, float dr, float beta) {
   p.type = 0;
   p.r[3] = t, p.r[0] = rx, p.r[1] = ry, p.r[2] = rz;
   p.n[3] = dr, p.f = 1.f, p.u.p.a = 0, p.u.p.b = 0;
   p.u.p.beta = beta, p.u.p.tau = 0;
+}
+
+void addp_clst(float rx, float ry, float rz, float t, unsigned long long n, float dr, float beta) {
+  clst_setup(rx, ry, rz, t, dr, beta);
   float qmas = 1.f;
   unsigned long long num = qmas > 0 && qmas < 1 ? bnldev(n, qmas) : n;
   addh(num);
 }
 
-void addp_mopo(float rx, float ry, float rz, float t, float light, float length, float beta, float tau,
-               float fr) {  // gpu/f2k.cxx:520-528
+// set p for a segment with light yield `light`, luminescence time tau and bare-track fraction
+// fr (gpu/f2k.cxx:520-528); returns the Poisson mean of its photon count
+static double mopo_setup(float rx, float ry, float rz, float t, float light, float length, float beta, float tau,
+                         float fr) {
   p.type = 0;
   p.r[3] = t, p.r[0] = rx, p.r[1] = ry, p.r[2] = rz;
   p.n[3] = length, p.f = fr, p.u.p.a = 0, p.u.p.b = 0;
   p.u.p.beta = beta, p.u.p.tau = tau;
-  addh(poidev(light * det.eff / det.ovr));
+  return light * det.eff / det.ovr;
+}
+
+void addp_mopo(float rx, float ry, float rz, float t, float light, float length, float beta, float tau, float fr) {
+  addh(poidev(mopo_setup(rx, ry, rz, t, light, length, beta, tau, fr)));
 }
 
 static void finc() {
@@ -729,6 +740,20 @@ const float* fldir() {
 ppc_hip_bunch flasher_bunch(int str, int dom) {
   flset(str, dom);
   return p;
+}
+
+ppc_hip_bunch clst_bunch(float rx, float ry, float rz, float t, float nx, float ny, float nz, float dr, float beta) {
+  p.n[0] = nx, p.n[1] = ny, p.n[2] = nz;
+  clst_setup(rx, ry, rz, t, dr, beta);
+  return p;
+}
+
+double mopo_bunch(float rx, float ry, float rz, float t, float nx, float ny, float nz, float light, float length,
+                  float beta, float tau, float fr, ppc_hip_bunch& out) {
+  p.n[0] = nx, p.n[1] = ny, p.n[2] = nz;
+  double mean = mopo_setup(rx, ry, rz, t, light, length, beta, tau, fr);
+  out = p;
+  return mean;
 }
 
 double particle_bunch(float rx, float ry, float rz, float t, float E, float dr, int type, float nx, float ny,

@@ -16,6 +16,10 @@ void list_devices();
 
 // bunch templates as the front-end builds them (used by the C facade)
 ppc_hip_bunch flasher_bunch(int str, int dom);
+// the bunches addp_clst() and addp_mopo() build for direction n (num left as set)
+ppc_hip_bunch clst_bunch(float rx, float ry, float rz, float t, float nx, float ny, float nz, float dr, float beta);
+double mopo_bunch(float rx, float ry, float rz, float t, float nx, float ny, float nz, float light, float length,
+                  float beta, float tau, float fr, ppc_hip_bunch& out);
 double particle_bunch(float rx, float ry, float rz, float t, float E, float dr, int type, float nx, float ny,
                       float nz, ppc_hip_bunch& out);
 

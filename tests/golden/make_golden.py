@@ -12,8 +12,10 @@ Fixtures (data only: inputs and the reference's outputs on them):
   replay_<case>.npz       single photons from random MWC states: state before emission,
                           state after transport, and every raw hit of each photon
   cli_c1.npz              HIT lines of the reference command line `ppc 1 30 N 0` (C1 flasher)
+  cli_c1ng.npz            the same in the 7-string ice with nextgen multi-PMT modules (HIT str dom_pmt)
   cli_f2k_c2.npz          HIT lines of `ppc k < TR line`, one 1 TeV EM cascade per process (86 strings)
-  cli_f2k_muon_c2.npz     the same for 128 down-going 1 TeV muons of 1 km (amu TR lines)
+  cli_f2k_muon_c2.npz     HIT lines of `ppc k < f2k` for 1 km down-going 1 TeV muons (amu TR lines):
+                          16 processes reading 16 TR lines each, with each HIT line's event
 The synthetic ice directories are regenerated from their seeds by ppc_amd.configs.
 """
 import ctypes
@@ -58,6 +60,13 @@ CASES = {
     "c2xho": ("c2x", {"HIFL": "300", "OVSZ": "3"}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
     # one whole 1 km down-going 1 TeV muon (the amu TR line of the command line's f2k stream test)
     "c3km": ("c2", {}, ("muon_km",)),
+    # the IceTray entry points (gpu/f2k.cxx:507-528): a 5 m segment of a particle at beta = 0.77,
+    # near the Cherenkov threshold (addp_clst); a 5 m segment with delayed isotropic
+    # luminescence, tau = 40 ns (addp_mopo); a point with beta = 0.85 and 30 % of its light in
+    # cascade directions (addp_mopo)
+    "c3clst": ("c2", {}, ("clst", dict(dr=5.0, beta=0.77))),
+    "c3mopo": ("c2", {}, ("mopo", dict(length=5.0, beta=1.0, tau=40.0, fr=0.5))),
+    "c3mopob": ("c2", {}, ("mopo", dict(length=0.0, beta=0.85, tau=0.0, fr=0.7))),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
@@ -100,6 +109,20 @@ def _bunch(T, src, num, det=None):
             b, _ = det.particle_bunch(s["x"], s["y"], s["z"], 0.0, s["E"], s["dr"], 0, n, num=num)
             return U.to_orc_bunch(b)
         b, _ = U.ref_particle(s["x"], s["y"], s["z"], 0.0, s["E"], s["dr"], 0, n)
+        b.num = num
+        return b
+    if src[0] in ("clst", "mopo"):  # at the 1 m muon segment's position and direction
+        s, a = C3_SEGMENT, src[1]
+        n = np.array(s["n"]) / np.linalg.norm(s["n"])
+        if src[0] == "clst":
+            if det is not None:
+                return U.to_orc_bunch(det.clst_bunch(s["x"], s["y"], s["z"], 0.0, n, a["dr"], a["beta"], num=num))
+            b = U.ref_clst(s["x"], s["y"], s["z"], 0.0, n, a["dr"], a["beta"])
+        else:
+            args = (s["x"], s["y"], s["z"], 0.0, n, 1.0, a["length"], a["beta"], a["tau"], a["fr"])
+            if det is not None:
+                return U.to_orc_bunch(det.mopo_bunch(*args, num=num)[0])
+            b = U.ref_mopo(*args)[0]
         b.num = num
         return b
     x, y, z, nx, ny = src[1:]
@@ -310,6 +333,30 @@ def make_cli(n=int(1.6e9)):
     print("cli c1: %d photons, %d HIT lines" % (photons, len(rows)))
 
 
+def make_cli_nextgen(n=int(8e9)):
+    """accepted HIT lines ("HIT str dom_pmt ...") of the reference command line on the C1 flasher
+    in the 7-string ice with nextgen multi-PMT modules on strings 1 and 2 (configs.write_c1ng)"""
+    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure("c1ng")})
+    procs = [subprocess.Popen([U.REF_EXE, "1", "30", "%d" % (n // 8), str(k)], env=env, stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True) for k in range(8)]
+    # one thread per process, so that the processes run side by side and none blocks on a full pipe
+    with ThreadPoolExecutor(len(procs)) as ex:
+        res = list(ex.map(lambda p: p.communicate(), procs))
+    photons, rows = 0, []
+    for out, err in res:
+        photons += sum(int(l.split()[1]) for l in err.splitlines() if l.startswith("photons:"))
+        for l in out.splitlines():
+            if l.startswith("HIT"):
+                f = l.split()
+                dom, pmt = f[2].split("_")
+                rows.append([float(f[1]), float(dom), float(pmt), float(f[3]), float(f[4])])
+    arr = np.array(rows, dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "cli_c1ng.npz"), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
+                        dom=arr[:, 1].astype(np.int16), pmt=arr[:, 2].astype(np.int16),
+                        t=arr[:, 3].astype(np.float32), wv=arr[:, 4].astype(np.float32))
+    print("cli c1ng: %d photons, %d HIT lines" % (photons, len(rows)))
+
+
 def cli_f2k_lines(events=8, E=1000.0):
     """f2k TR lines of EM cascades on the 86-string ice (the benchmark cascade's position and
     direction, energy E GeV): the input of the f2k mode of the command line"""
@@ -346,6 +393,32 @@ def make_cli_f2k(name="cli_f2k_c2.npz", lines=None):
     print("%s: %d photons, %d HIT lines" % (name, photons, len(rows)))
 
 
+def make_cli_f2k_events(name, lines, procs, sample=200000):
+    """accepted HIT lines of the reference command line in its f2k mode, `procs` processes
+    each reading the same stream of TR lines (`ppc k < f2k`, k = 0..procs-1), with the event
+    (TR line) of every HIT line, so that tests can compare samples event by event"""
+    env = _env({"PPCTABLESDIR": configs.ensure("c2")})
+    ps = [subprocess.Popen([U.REF_EXE, str(k)], env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True) for k in range(procs)]
+    with ThreadPoolExecutor(len(ps)) as ex:
+        res = list(ex.map(lambda p: p.communicate("".join(lines)), ps))
+    photons = sum(int(x.split()[1]) for _, err in res for x in err.splitlines() if x.startswith("photons:"))
+    rows, ev, e = [], [], -1
+    for out, _ in res:
+        for l in out.splitlines():
+            if l.startswith("TR"):
+                e += 1
+            elif l.startswith("HIT"):
+                rows.append(l.split()[1:5])
+                ev.append(e)
+    arr = np.array(rows, dtype=np.float64)
+    sub = np.sort(np.random.default_rng(3).permutation(len(arr))[:sample])
+    np.savez_compressed(os.path.join(HERE, name), photons=np.int64(photons), events=np.int64(e + 1),
+                        ev=np.array(ev, dtype=np.int16), str=arr[:, 0].astype(np.int16), dom=arr[:, 1].astype(np.int16),
+                        t=arr[sub, 2].astype(np.float32), wv=arr[sub, 3].astype(np.float32))
+    print("%s: %d photons, %d events, %d HIT lines" % (name, photons, e + 1, len(arr)))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 5 and sys.argv[1] == "_worker":
         fn = {"stats": _stats_worker, "replay": _replay_worker}[sys.argv[2]]
@@ -369,7 +442,10 @@ if __name__ == "__main__":
             make_replay(c)
     if "cli" in what:
         make_cli()
+    if "cli" in what or "cli_nextgen" in what:
+        make_cli_nextgen()
     if "cli" in what or "cli_f2k" in what:
         make_cli_f2k()
     if "cli" in what or "cli_f2k_muon" in what:
-        make_cli_f2k("cli_f2k_muon_c2.npz", cli_f2k_muon_lines())
+        # 16 processes, each reading a stream of 16 muon TR lines
+        make_cli_f2k_events("cli_f2k_muon_c2.npz", cli_f2k_muon_lines(events=16), 16)

@@ -374,6 +374,25 @@ def ref_particle(x, y, z, t, E, dr, ptype, n):
     return b, mean
 
 
+def ref_clst(x, y, z, t, n, dr, beta):
+    """the reference's addp_clst() bunch for direction n"""
+    lib = reference()
+    lib.ref_clst.argtypes = [ctypes.c_float] * 9 + [ctypes.POINTER(Bunch)]
+    b = Bunch()
+    lib.ref_clst(x, y, z, t, n[0], n[1], n[2], dr, beta, ctypes.byref(b))
+    return b
+
+
+def ref_mopo(x, y, z, t, n, light, length, beta, tau, fr):
+    """the reference's addp_mopo() bunch for direction n and its mean photon count"""
+    lib = reference()
+    lib.ref_mopo.restype = ctypes.c_double
+    lib.ref_mopo.argtypes = [ctypes.c_float] * 12 + [ctypes.POINTER(Bunch)]
+    b = Bunch()
+    mean = lib.ref_mopo(x, y, z, t, n[0], n[1], n[2], light, length, beta, tau, fr, ctypes.byref(b))
+    return b, mean
+
+
 def ref_flasher(string, dom):
     lib = reference()
     lib.ref_flasher.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Bunch)]

@@ -38,3 +38,24 @@ def test_accept_tables_follow_the_detector(c1_dir, c2_dir):
         for k in range(T.nsensor):
             S = T.sensors[k]
             assert (S.def_ == 1 and 1 <= S.ns <= 16) or (S.def_ == 0 and S.npmt >= 1)
+
+
+def test_accept_tables_of_multi_pmt_modules():
+    """the nextgen ice (om.conf: a 12-PMT and a 2-PMT module on strings 1 and 2): their DOMs
+    point to sensors with those PMT counts and the modules' radii and axis ratios; the other
+    DOMs keep the one-PMT angular-sensitivity sensor"""
+    from ppc_amd import configs
+
+    det, T = _tables(configs.ensure("c1ng"), {"WFLA": "405", "FWID": "10"})
+    gsize = int(det.scalar("gsize", np.int32))
+    strs = det.array("str", np.int32)
+    doms = [T.doms[i] for i in range(gsize)]
+    for s, npmt, R, Rz in ((1, 12, 0.16, 0.205), (2, 2, 0.15, 0.27)):
+        sel = [d for d, k in zip(doms, strs) if k == s]
+        assert len(sel) == 60
+        S = T.sensors[sel[0].sensor]
+        assert S.def_ == 0 and S.npmt == npmt
+        assert all(d.sensor == sel[0].sensor for d in sel)
+        assert np.allclose([d.R for d in sel], R) and np.allclose([d.F for d in sel], Rz / R, rtol=1e-6)
+    other = [d for d, k in zip(doms, strs) if k > 2]
+    assert all(T.sensors[d.sensor].def_ == 1 for d in other)

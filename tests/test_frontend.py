@@ -1,7 +1,8 @@
 """The product's front-end (ppc_amd/host/frontend.cpp) against the reference's
-own: the bunches addp() and flset() build (gpu/f2k.cxx:17-167, 484-504,
-630-759) for every particle type and for flashers under the beam-shaping
-environment variables, compared field by field with the reference's CPU build."""
+own: the bunches addp(), addp_clst(), addp_mopo() and flset() build
+(gpu/f2k.cxx:17-167, 484-528, 630-759) for every particle type, for the IceTray
+entry points and for flashers under the beam-shaping environment variables,
+compared field by field with the reference's CPU build."""
 import math
 import os
 import subprocess
@@ -39,6 +40,23 @@ for ptype in [0, 1, 11, 12, 13, 14]:
         if not np.allclose(fr, fo, rtol=2.5e-7, atol=0) or abs(rmean - pmean) > 1e-6 * max(1.0, abs(rmean)):
             bad += 1
             print("MISMATCH", ptype, U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
+for k in range(20):  # the IceTray entry points addp_clst / addp_mopo (gpu/f2k.cxx:507-528)
+    x, y, z, t = rng.uniform(-300, 300, 4)
+    n = rng.normal(size=3); n /= np.linalg.norm(n)
+    dr, beta = float(rng.choice([0.0, 2.5])), float(rng.uniform(0.6, 1.0))
+    rb, ob = U.ref_clst(x, y, z, t, n, dr, beta), U.to_orc_bunch(D.clst_bunch(x, y, z, t, n, dr, beta))
+    rb.q = ob.q = 0
+    if U.bunch_fields(rb) != U.bunch_fields(ob):
+        bad += 1
+        print("MISMATCH clst", U.bunch_fields(rb), U.bunch_fields(ob))
+    light, tau, fr = float(rng.uniform(0, 500)), float(rng.choice([0.0, 30.0])), float(rng.uniform(0, 1))
+    rb, rmean = U.ref_mopo(x, y, z, t, n, light, dr, beta, tau, fr)
+    pb, pmean = D.mopo_bunch(x, y, z, t, n, light, dr, beta, tau, fr, num=0)
+    ob = U.to_orc_bunch(pb)
+    rb.q = ob.q = 0
+    if U.bunch_fields(rb) != U.bunch_fields(ob) or abs(rmean - pmean) > 1e-6 * max(1.0, abs(rmean)):
+        bad += 1
+        print("MISMATCH mopo", U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
 for s, d in [(1, 30), (2, 10), (0, 1), (0, 2), (-3, 5)]:
     rb = U.ref_flasher(s, d)
     pb = U.to_orc_bunch(D.flasher_bunch(s, d))

@@ -129,3 +129,50 @@ def test_clearance_skip_changes_no_hit(det):
         dev.close()
     assert len(out[0][0]) > 100
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+
+
+def test_full_c4_batch_graph_equals_eager(c1_dir):
+    """The whole batched workload on the 7-string ice -- 512 sources (flashers, cascades, 10 m
+    muon segments) with photon counts log-uniform in [1e5, 1e8], 8 ice variants, 8.1e9 photons
+    in groups of at most 3e9 (32-bit photon indices), one context per group -- launched once
+    eagerly and once as a replay of its captured graph from the same lane states: the hits
+    are identical, hit for hit, every photon of every source runs, and each source's hits
+    carry its own segment only."""
+    from ppc_amd import api, configs
+
+    det = api.Detector(c1_dir, {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+    bunches, variant, layers = configs.c4_batch(det, seed=5)
+    groups, cur, tot = [], [], 0
+    for k, b in enumerate(bunches):
+        if cur and tot + b.num > 3e9:
+            groups.append(cur)
+            cur, tot = [], 0
+        cur.append(k)
+        tot += b.num
+    groups.append(cur)
+    total_hits = 0
+    for gi, g in enumerate(groups):
+        bs = [bunches[k] for k in g]
+        num = sum(b.num for b in bs)
+        print("C4 group %d: %d sources, %.3g photons" % (gi, len(g), num), flush=True)
+        dev = api.Device(0, deterministic=True, max_bunches=len(g), max_hits=1 << 25)
+        dev.upload(det)
+        dev.set_layers(layers)
+        S = _states(dev.lanes, 40 + gi)
+        dev.set_states(S)
+        dev.launch_batch(bs, variant[g])
+        st = dev.wait()
+        assert st.photons == num and st.stored == st.hits
+        he, pe = _sorted_hits(dev)
+        dev.graph_capture(bs, variant[g])
+        dev.set_states(S)
+        dev.graph_launch()
+        st2 = dev.wait()
+        hg, pg = _sorted_hits(dev)
+        dev.close()
+        assert st2.photons == num and st2.hits == st.hits
+        assert np.array_equal(pe, pg) and np.array_equal(he, hg)
+        assert set(np.unique(he["n"]).tolist()) <= set(g)
+        total_hits += st.hits
+    assert total_hits > 100000
+    print("C4: %d groups, %d hits, graph replays equal to eager launches" % (len(groups), total_hits), flush=True)

@@ -66,27 +66,99 @@ def test_cli_flasher_matches_reference_cli():
     _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3])
 
 
-@pytest.mark.parametrize("kind", [
-    "cascade",
-    pytest.param("muon", marks=pytest.mark.xfail(strict=False, reason=(
-        "open: HIT rate, time and wavelength agree, but the per-DOM counts of DOMs 100-150 m from the "
-        "1 km track differ beyond chance (two-sample chi2 581/474 against 3.2e8 reference photons; each "
-        "sample self-consistent on half splits); cause not yet found"))),
-])
-def test_cli_f2k_stream_matches_reference_cli(kind):
-    """`ppc 0 < events.f2k` on the 86-string ice against the reference command line's HIT lines
-    on the same TR lines: 1 TeV EM cascades (tests/golden/cli_f2k_c2.npz, 8 events, 8e7
-    photons) or 1 km down-going 1 TeV muons (cli_f2k_muon_c2.npz, 128 events, 3.2e8 photons);
-    8e8 photons here, in one process"""
+@pytest.mark.parametrize("accept", ["host", "device"])
+def test_cli_nextgen_flasher_matches_reference_cli(accept):
+    """`ppc 1 30 N 0` in the 7-string ice with nextgen multi-PMT modules (a 12-PMT module on
+    string 1, a 2-PMT module on string 2, ellipsoidal; ini.cxx:731-761) against the reference
+    command line's "HIT str dom_pmt" lines (tests/golden/cli_c1ng.npz): HIT rate, counts per
+    (string, DOM, PMT), arrival times and wavelengths; with print()'s acceptance on the host
+    (the reference's) and on the device (PPC_ACCEPT=device)"""
+    from ppc_amd import _native, configs
+
+    env = _env()
+    env["PPCTABLESDIR"] = configs.ensure("c1ng")
+    if accept == "device":
+        env["PPC_ACCEPT"] = "device"
+    print("nextgen flasher, acceptance on the %s" % accept, flush=True)
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "1.6e10", "0"], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    photons = sum(int(m) for m in re.findall(r"photons: (\d+)", r.stderr))
+    rows = []
+    for l in r.stdout.splitlines():
+        if l.startswith("HIT "):
+            f = l.split()
+            dom, pmt = f[2].split("_")
+            rows.append((int(f[1]), int(dom), int(pmt), float(f[3]), float(f[4])))
+    arr = np.array(rows)
+    gold = np.load(os.path.join(HERE, "golden", "cli_c1ng.npz"))
+    Nr, nr = int(gold["photons"]), len(gold["t"])
+    z = (len(arr) / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + len(arr) / photons ** 2)
+    p_rate = 2 * stats.norm.sf(abs(z))
+    key = lambda s, d, p: (np.asarray(s, np.int64) * 100 + np.asarray(d, np.int64)) * 100 + np.asarray(p, np.int64)
+    kg, kp = key(gold["str"], gold["dom"], gold["pmt"]), key(arr[:, 0], arr[:, 1], arr[:, 2])
+    allk = np.union1d(kg, kp)
+    chi2, dof, p_bin = _two_sample_chi2(np.array([np.sum(kp == k) for k in allk]),
+                                        np.array([np.sum(kg == k) for k in allk]))
+    p_t = stats.ks_2samp(arr[:, 3], gold["t"]).pvalue
+    p_w = stats.ks_2samp(arr[:, 4], gold["wv"]).pvalue
+    multi = arr[:, 0] <= 2
+    print("HIT rate %.4e vs %.4e (p=%.3f), (DOM, PMT) chi2 %.1f/%d (p=%.3f), KS time p=%.3f, wavelength p=%.3f; "
+          "%d hits on multi-PMT modules, PMTs %s" % (len(arr) / photons, nr / Nr, p_rate, chi2, dof, p_bin, p_t, p_w,
+                                                    multi.sum(), np.unique(arr[multi, 2]).tolist()))
+    assert multi.sum() > 1000 and len(np.unique(arr[arr[:, 0] == 1, 2])) > 6
+    assert p_rate > 0.01 and p_bin > 0.01 and p_t > 0.01 and p_w > 0.01
+
+
+def _event_counts(ev, strs, doms, nev, gsize, index):
+    """hits per event and DOM (events x DOMs) from the HIT lines' events and (string, DOM)"""
+    C = np.zeros((nev, gsize), dtype=np.int64)
+    np.add.at(C, (np.asarray(ev, np.int64), index(strs, doms)), 1)
+    return C
+
+
+def _permutation_chi2(E1, E2, nperm=2000, seed=1, min_exp=10):
+    """Two-sample chi-square of the per-DOM counts of two sets of events (bins merged as in
+    _two_sample_chi2), and its p-value under the permutations of whole events between the
+    two sets: a photon can hit several DOMs, so DOM counts are not independent Poisson
+    variables and the chi-square distribution understates the statistic's spread; events are
+    independent and exchangeable under the null, so their permutations give its law"""
+    a, b = E1.sum(axis=0).astype(float), E2.sum(axis=0).astype(float)
+    A, B = a.sum(), b.sum()
+    groups, cur, ca, cb = [], [], 0.0, 0.0
+    for i in np.argsort(-(a + b)):
+        cur.append(i)
+        ca += a[i]
+        cb += b[i]
+        tot = ca + cb
+        if tot * A / (A + B) >= min_exp and tot * B / (A + B) >= min_exp:
+            groups.append(cur)
+            cur, ca, cb = [], 0.0, 0.0
+    if cur and groups:
+        groups[-1] += cur
+    E = np.concatenate([E1, E2])
+    M = np.column_stack([E[:, q].sum(axis=1) for q in groups]).astype(float)
+    total = M.sum(axis=0)
+
+    def stat(sel):
+        xa = M[sel].sum(axis=0)
+        xb = total - xa
+        k1, k2 = math.sqrt(xb.sum() / xa.sum()), math.sqrt(xa.sum() / xb.sum())
+        return float((((k1 * xa - k2 * xb) ** 2) / (xa + xb)).sum())
+
+    sel0 = np.zeros(len(E), dtype=bool)
+    sel0[:len(E1)] = True
+    obs = stat(sel0)
+    rng = np.random.default_rng(seed)
+    null = np.array([stat(rng.permutation(sel0)) for _ in range(nperm)])
+    return obs, len(groups) - 1, float((1 + (null >= obs).sum()) / (nperm + 1))
+
+
+def _run_f2k(lines):
     from ppc_amd import _native, configs
 
     env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS}
     env.update({"PPCTABLESDIR": configs.ensure("c2"), "DTMN": "1"})
-    if kind == "cascade":
-        lines, fixture = G.cli_f2k_lines(events=80), "cli_f2k_c2.npz"
-    else:
-        lines, fixture = G.cli_f2k_muon_lines(events=320), "cli_f2k_muon_c2.npz"
-    print("f2k stream: %d %s TR lines on c2" % (len(lines), kind), flush=True)
     r = subprocess.run([_native.ppc_exe(), "0"], env=env, input="".join(lines), capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -94,10 +166,56 @@ def test_cli_f2k_stream_matches_reference_cli(kind):
     out = r.stdout.splitlines()
     # every input line is echoed, and the HIT lines follow the events
     assert sum(l.startswith("TR ") for l in out) == len(lines)
-    rows = [l.split() for l in out if l.startswith("HIT ")]
-    arr = np.array([[float(v) for v in row[1:5]] for row in rows])
+    rows, ev, e = [], [], -1
+    for l in out:
+        if l.startswith("TR "):
+            e += 1
+        elif l.startswith("HIT "):
+            rows.append([float(v) for v in l.split()[1:5]])
+            ev.append(e)
+    return photons, np.array(rows), np.array(ev)
+
+
+def test_cli_f2k_cascade_stream_matches_reference_cli():
+    """`ppc 0 < events.f2k` on the 86-string ice against the reference command line's HIT lines
+    on the same TR lines: 1 TeV EM cascades (tests/golden/cli_f2k_c2.npz, 8 events, 8e7
+    photons); 8e8 photons here, in one process"""
+    lines = G.cli_f2k_lines(events=80)
+    print("f2k stream: %d cascade TR lines on c2" % len(lines), flush=True)
+    photons, arr, _ = _run_f2k(lines)
     assert photons > 5e8
-    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture=fixture)
+    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_f2k_c2.npz")
+
+
+def test_cli_f2k_muon_stream_matches_reference_cli():
+    """`ppc 0 < events.f2k` with 1 km down-going 1 TeV muons (amu TR lines) on the 86-string ice
+    against the reference command line on the same TR lines (tests/golden/cli_f2k_muon_c2.npz:
+    16 processes reading 16 TR lines each, 6.5e8 photons); 320 events, 8e8 photons here, in
+    one process.  HIT rate (Poisson), per-DOM counts (two-sample chi-square, p-value from
+    permutations of whole events: DOM counts of a muon are correlated, since 6 % of its hits
+    come from photons that hit more than once), arrival-time and wavelength distributions (KS)"""
+    from ppc_amd import api, configs
+
+    lines = G.cli_f2k_muon_lines(events=320)
+    print("f2k stream: %d muon TR lines on c2" % len(lines), flush=True)
+    photons, arr, ev = _run_f2k(lines)
+    assert photons > 5e8
+    gold = np.load(os.path.join(HERE, "golden", "cli_f2k_muon_c2.npz"))
+    Nr, nr = int(gold["photons"]), len(gold["ev"])
+    z = (len(arr) / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + len(arr) / photons ** 2)
+    p_rate = 2 * stats.norm.sf(abs(z))
+    det = api.Detector(configs.ensure("c2"), {"DTMN": "1"})
+    key = np.asarray(det.array("str", np.int32), np.int64) * 100 + det.array("dom", np.int32)
+    pos = {int(k): i for i, k in enumerate(key)}
+    index = lambda s, d: np.array([pos[int(a) * 100 + int(b)] for a, b in zip(s, d)], dtype=np.int64)
+    E1 = _event_counts(ev, arr[:, 0], arr[:, 1], len(lines), len(key), index)
+    E2 = _event_counts(gold["ev"], gold["str"], gold["dom"], int(gold["events"]), len(key), index)
+    chi2, dof, p_dom = _permutation_chi2(E1, E2)
+    p_t = stats.ks_2samp(arr[:, 2], gold["t"]).pvalue
+    p_w = stats.ks_2samp(arr[:, 3], gold["wv"]).pvalue
+    print("HIT rate %.4e vs %.4e (p=%.3f), DOM chi2 %.1f/%d (permutation p=%.3f), KS time p=%.3f, wavelength p=%.3f"
+          % (len(arr) / photons, nr / Nr, p_rate, chi2, dof, p_dom, p_t, p_w))
+    assert p_rate > 0.01 and p_dom > 0.01 and p_t > 0.01 and p_w > 0.01
 
 
 _LIB = r'''

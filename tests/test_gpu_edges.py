@@ -108,3 +108,55 @@ def test_bad_launch_shapes_are_rejected():
     dev = api.Device(0)  # and a good one still opens afterwards
     assert dev.lanes > 0
     dev.close()
+
+
+def test_nonfinite_layer_guard_like_the_oracle(det):
+    """A NaN forced from the host into one layer's absorption factor: a photon whose optical
+    depth turns non-finite ends in that step and is counted (the guard of the reference's GPU
+    build, gpu/pro.cu:727-729), every launch completes, and photon by photon the lanes end in
+    the oracle's MWC states (same draws, the guard at the same step)."""
+    import ctypes
+
+    import oracle_util as U
+    from ppc_amd import api
+
+    src = (2.0, 1.0, 3.0)
+    hmin, rdh = float(det.scalar("hmin")), float(det.scalar("rdh"))
+    J = int(round((src[2] - hmin) * rdh)) + 3  # 30 m above the source
+    lay = det.layers()
+    lay["p"][J] = np.nan
+    dev = api.Device(0, deterministic=True)
+    dev.upload(det)
+    dev.set_layers(lay)
+    rng = np.random.default_rng(31)
+    x = rng.integers(1, 2 ** 32, size=dev.lanes, dtype=np.uint64)
+    c = rng.integers(0, 2 ** 30, size=dev.lanes, dtype=np.uint64)
+    dev.set_states(x | (c << np.uint64(32)))
+    N = dev.lanes  # one photon per lane
+    print("NaN absorption in layer %d, %d photons" % (J, N), flush=True)
+    st = dev.run([U.to_api_bunch(U.point_bunch(*src, num=N))])
+    fin = dev.states()
+    dev.close()
+    assert st.photons == N
+    # the oracle on the first photons, with the same NaN in its tables
+    T = U.product_tables(det)
+    ab = np.frombuffer(bytes(T.field("abs_")), np.float32).reshape(U.WNUM, U.MAXLYS).copy()
+    ab[:, J] = np.nan
+    ctypes.memmove(T.field("abs_"), ab.tobytes(), ab.nbytes)
+    rm = det.array("rm", np.uint32)
+    lib = U.oracle(nan=True)
+    b = U.point_bunch(*src, num=N)
+    H = (U.Hit * 64)()
+    n = 20000
+    cnt = U.Counters()
+    same = 0
+    for k in range(n):
+        s = U.Rng(int(x[k]), int(c[k]), int(rm[k]))
+        lib.orc_photon(T.ptr, ctypes.byref(b), 1, k, ctypes.byref(s), H, 64, ctypes.byref(cnt))
+        same += (s.x, s.c) == (int(fin[k]) & 0xFFFFFFFF, int(fin[k]) >> 32)
+    f_gpu, f_orc = st.nonfinite / N, cnt.nonfinite / n
+    print("non-finite: GPU %d of %d (%.4f), oracle %d of %d (%.4f); %d of %d lanes in the oracle's state"
+          % (st.nonfinite, N, f_gpu, cnt.nonfinite, n, f_orc, same, n), flush=True)
+    assert cnt.nonfinite > 100 and st.nonfinite > 0
+    assert same >= 0.99 * n
+    assert abs(f_gpu - f_orc) < 5 * np.sqrt(f_orc * (1 - f_orc) / n)
