@@ -277,6 +277,19 @@ int ppc_hip_adjoint(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb,
                     const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
                     const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* stats);
 
+/* ppc_hip_adjoint without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight fitting
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as ppc_hip_adjoint;
+ * its staging (records, weights, d(p)/d(a), d(s1)/d(b)), gradient accumulators and counters
+ * live in a fixed scratch area of the GPU (2 MiB, set up by ppc_hip_open) instead of buffers
+ * the context allocates on first use.  Limits: nseg * gsize <= 262144 weights and, when
+ * rec is given, nrec <= 65536 records; larger calls fail with PPC_HIP_E_CAPACITY.  Every
+ * call on one GPU uses the same scratch area (each call clears what must start at zero):
+ * calls that overlap on one GPU, from two contexts or two threads, race. */
+int ppc_hip_adjoint_noalloc(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                            const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                            const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* stats);
+
 /* ---- hit post-processing on the device ---- */
 /* sensor type: angular sensitivity and PMT layout (gpu/ini.cxx:122-281) */
 typedef struct {

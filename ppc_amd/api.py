@@ -107,6 +107,7 @@ def hip():
         lib.ppc_hip_accept.argtypes = [c_p, ctypes.c_uint64, c_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         lib.ppc_hip_adjoint.argtypes = [c_p, ctypes.POINTER(Bunch), ctypes.c_uint32, ctypes.c_uint32, c_p,
                                         ctypes.c_uint32, c_p, ctypes.c_uint32, c_p, c_p, c_p, ctypes.POINTER(Stats)]
+        lib.ppc_hip_adjoint_noalloc.argtypes = lib.ppc_hip_adjoint.argtypes
         _hip = lib
     return _hip
 
@@ -437,11 +438,12 @@ class Device:
         _check(self.lib.ppc_hip_hit_photons(self.ctx, a.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
         return a[:n.value]
 
-    def adjoint(self, bunches, num, weight, dpda, ds1db, records=None):
+    def adjoint(self, bunches, num, weight, dpda, ds1db, records=None, noalloc=False):
         """replay photons of a forward launch of these bunches -- ``records``
         (ADJ_RECORD_DTYPE), or by default the hitting photons the last launch
         recorded -- and return (d/d adust[size], d/d be[size], stats);
-        weight [nseg][gsize] is d(LLH)/d(mean hits)"""
+        weight [nseg][gsize] is d(LLH)/d(mean hits).  noalloc: the entry point that
+        allocates no device memory (ppc_hip_adjoint_noalloc)"""
         arr, _ = self._bunches(bunches, None)
         weight = np.ascontiguousarray(weight, dtype=np.float32)
         nseg = weight.shape[0] if weight.ndim == 2 else 1
@@ -455,7 +457,8 @@ class Device:
             rp, nrec = records.ctypes.data_as(ctypes.c_void_p), len(records)
         else:
             rp, nrec = None, 0
-        _check(self.lib.ppc_hip_adjoint(self.ctx, arr, len(arr), num, rp, nrec,
+        fn = self.lib.ppc_hip_adjoint_noalloc if noalloc else self.lib.ppc_hip_adjoint
+        _check(fn(self.ctx, arr, len(arr), num, rp, nrec,
                                         weight.ctypes.data_as(ctypes.c_void_p), nseg,
                                         dpda.ctypes.data_as(ctypes.c_void_p), ds1db.ctypes.data_as(ctypes.c_void_p),
                                         grad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))

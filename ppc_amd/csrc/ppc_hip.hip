@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,41 @@ int fail(int code, const char* fmt, ...) {
 template <class T>
 hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, sizeof(T) * (n > 0 ? n : 1));
+}
+
+// ---- the scratch area of ppc_hip_adjoint_noalloc ----
+// One per GPU, as a file-scope device array, so that the entry point allocates nothing.
+// Host code never passes the array's name to a kernel or a copy (on the host it is only a
+// stand-in for the symbol): ppc_hip_open looks up its device address once per GPU with
+// hipGetSymbolAddress and every later use goes through that pointer.  Calls on one GPU
+// share the area: overlapping calls (two contexts on one GPU, two host threads) race.
+constexpr uint32_t kScratchWeights = 1u << 18;  // LLH weights: segments x DOMs <= 262144 (1 MiB)
+constexpr uint32_t kScratchRecords = 1u << 16;  // replay records the caller passes (1 MiB)
+struct AdjScratch {
+  double grad[2 * PPC_HIP_MAXLYS];
+  float dp[2 * PPC_HIP_MAXLYS];
+  ppc::Counters ctr;
+  float weight[kScratchWeights];
+  ppc_hip_adj_record rec[kScratchRecords];
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(AdjScratch)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+AdjScratch* g_scratch[kMaxDevices] = {};
+
+// device address of this GPU's scratch area (the current device must be `device`)
+hipError_t scratch_address(int device, AdjScratch** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<AdjScratch*>(ptr);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
 }
 
 __global__ void zero_words(uint32_t* p, uint32_t n) {
@@ -131,6 +167,7 @@ struct ppc_hip_ctx {
   size_t adj_wcap = 0;
   float* d_adj_dp = nullptr;
   double* d_adj_grad = nullptr;
+  AdjScratch* scratch = nullptr;  // this GPU's scratch area (ppc_hip_adjoint_noalloc)
   uint64_t photon_seed = 0;       // ppc_hip_seed_photons
   hipGraph_t graph = nullptr;  // ppc_hip_graph_capture: the graph and the buffers it reads
   hipGraphExec_t graph_exec = nullptr;
@@ -261,6 +298,11 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
     int code = e == hipErrorOutOfMemory ? PPC_HIP_E_NOMEM : PPC_HIP_E_DEVICE;
     ppc_hip_close(c);
     return fail(code, "allocation failed: %s", hipGetErrorString(e));
+  }
+  e = scratch_address(device, &c->scratch);
+  if (e != hipSuccess) {
+    ppc_hip_close(c);
+    return fail(PPC_HIP_E_DEVICE, "scratch area of device %d: %s", device, hipGetErrorString(e));
   }
   *out = c;
   return PPC_HIP_OK;
@@ -889,6 +931,23 @@ int ppc_hip_adjoint(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, u
   if (!c->d_adj_dp) HIPCHK(dalloc(&c->d_adj_dp, 2 * PPC_HIP_MAXLYS));
   if (!c->d_adj_grad) HIPCHK(dalloc(&c->d_adj_grad, 2 * PPC_HIP_MAXLYS));
   AdjBuffers B{c->d_adj_rec, c->d_adj_w, c->d_adj_dp, c->d_adj_grad, c->d_ctr_adj};
+  return adjoint_run(c, B, bunches, nb, num, rec, nrec, weight, nseg, dpda, ds1db, grad, st);
+}
+
+int ppc_hip_adjoint_noalloc(ppc_hip_ctx* c, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num,
+                            const ppc_hip_adj_record* rec, uint32_t nrec, const float* weight, uint32_t nseg,
+                            const float* dpda, const float* ds1db, double* grad, ppc_hip_stats* st) {
+  int rc = adjoint_check(c, bunches, nb, num, rec, nrec, weight, nseg, dpda, ds1db, grad);
+  if (rc != PPC_HIP_OK) return rc;
+  if (!c->scratch) return fail(PPC_HIP_E_STATE, "no scratch area on device %d", c->device);
+  if ((uint64_t)std::max<uint32_t>(nseg, 1) * c->P.gsize > kScratchWeights)
+    return fail(PPC_HIP_E_CAPACITY, "%u segments x %d DOMs of weights exceed the scratch area's %u", nseg,
+                c->P.gsize, kScratchWeights);
+  if (rec && nrec > kScratchRecords)
+    return fail(PPC_HIP_E_CAPACITY, "%u replay records exceed the scratch area's %u", nrec, kScratchRecords);
+  HIPCHK(hipSetDevice(c->device));
+  AdjScratch* S = c->scratch;
+  AdjBuffers B{S->rec, S->weight, S->dp, S->grad, &S->ctr};
   return adjoint_run(c, B, bunches, nb, num, rec, nrec, weight, nseg, dpda, ds1db, grad, st);
 }
 
