@@ -57,13 +57,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 }
 
 // Multiply-with-carry: t = a*x + c (one v_mad_u64_u32), x = lo(t), c = hi(t);
-// uniform in (0,1] from the top 23 bits of x (gpu/pro.cu:47-57).
+// uniform in (0,1] from the top 23 bits of x (gpu/pro.cu:47-57).  x and c leave the
+// multiply as two 32-bit values (the empty asm): kept as one 64-bit register pair they
+// are spilled and reloaded as a pair, which the gfx950 register allocator has placed at
+// odd register numbers (an illegal encoding that fails the build).
 struct Mwc {
   uint32_t x, c, a;
   __device__ __forceinline__ float operator()() {
     uint64_t t = (uint64_t)a * x + c;
     x = (uint32_t)t;
     c = (uint32_t)(t >> 32);
+    asm("" : "+v"(x), "+v"(c));  // two 32-bit values from here on, not a register pair
     return 2.f - __uint_as_float((x >> 9) | 0x3f800000u);
   }
 };
@@ -117,25 +121,20 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
 }
 
 // Rotate unit n by the polar angle with 1-cos = cs, sin = si, around a uniform
-// azimuth (gpu/pro.cu:111-153); the largest component is swapped to x first.
+// azimuth (gpu/pro.cu:111-153); the largest component is swapped to x first, and
+// swapped back at the end.  The swaps are selects, so that lanes with different
+// largest components do not diverge (ties resolve as in the reference: y only if
+// it exceeds both others, z only if it exceeds x and is not below y).
 // NORM = false leaves out the final renormalisation, for callers whose next
 // operation is a linear map followed by a normalisation (which absorbs it).
 template <bool NORM = true>
 __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
-  float nx = n.x, ny = n.y, nz = n.z;
-  float r0 = nx * nx, r1 = ny * ny, r2 = nz * nz;
-  int ax = 0;
-  if (r1 > r2) {
-    if (r1 > r0) {
-      float t = nx; nx = ny; ny = t;
-      t = r0; r0 = r1; r1 = t;
-      ax = 1;
-    }
-  } else if (r2 > r0) {
-    float t = nx; nx = nz; nz = t;
-    t = r0; r0 = r2; r2 = t;
-    ax = 2;
-  }
+  const float x = n.x, y = n.y, z = n.z;
+  const float q0 = x * x, q1 = y * y, q2 = z * z;
+  const bool sy = (q1 > q2) & (q1 > q0);   // x and y trade places
+  const bool sz = !(q1 > q2) & (q2 > q0);  // x and z trade places
+  const float nx = sy ? y : sz ? z : x, ny = sy ? x : y, nz = sz ? x : z;
+  const float r0 = sy ? q1 : sz ? q2 : q0, r1 = sy ? q0 : q1, r2 = sz ? q0 : q2;
   // the perpendicular pair p1 = a (-ny, nx, 0), p2 = b (-nz, 0, nx) (unit: a, b normalise),
   // turned by 45 degrees into q = (p1 - p2) / |p1 - p2| and p = (p1 + p2) / |p1 + p2|; with
   // p1.p2 = ny nz a b = cc / 2 the norms are sqrt(2 - cc) and sqrt(2 + cc).  The new direction
@@ -148,17 +147,13 @@ __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
   const float c1 = si * __builtin_amdgcn_cosf(u) * frsq(2 - cc);
   const float c2 = si * __builtin_amdgcn_sinf(u) * frsq(2 + cc);
   const float E = a * (c1 + c2), F = b * (c1 - c2), omc = 1 - cs;
-  const float mx = nx * omc + (nz * F - ny * E);
-  const float my = ny * omc + nx * E;
-  const float mz = nz * omc - nx * F;
-  nx = mx, ny = my, nz = mz;
-  if (NORM) normalize3(nx, ny, nz);
-  if (ax == 1) {
-    float t = nx; nx = ny; ny = t;
-  } else if (ax == 2) {
-    float t = nx; nx = nz; nz = t;
-  }
-  n.x = nx, n.y = ny, n.z = nz;
+  float mx = nx * omc + (nz * F - ny * E);
+  float my = ny * omc + nx * E;
+  float mz = nz * omc - nx * F;
+  if (NORM) normalize3(mx, my, mz);
+  n.x = sy ? my : sz ? mz : mx;
+  n.y = sy ? mx : my;
+  n.z = sz ? mx : mz;
 }
 
 // ---------------------------------------------------------------- tables
@@ -514,9 +509,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   // one atomic claims a chunk, so the global counter sees one request per chunk
   uint32_t res_next = 0, res_end = 0;
   const uint32_t chunk = P.chunk;
-  // steps, photons and killed photons of the wave (uniform: scalar registers), non-finite
-  // photons of the lane (rare), event counts of the lane (COUNT builds only)
-  uint32_t n_steps = 0, n_photons = 0, n_killed = 0, n_nonfinite = 0;
+  // steps, photons and killed photons of the wave (uniform: scalar registers), event counts
+  // of the lane (COUNT builds only); non-finite photons (rare) go straight to the launch counter
+  uint32_t n_steps = 0, n_photons = 0, n_killed = 0;
   uint32_t n_cross = 0, n_str = 0, n_dom = 0, n_scat = 0;
   // ADJ: each record's photon runs twice from its start state: pass 0 sums the LLH
   // weights of its hits (Gsum), pass 1 scores the path with G, the weight of the hits
@@ -859,7 +854,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
 
     r.x += del * n.x, r.y += del * n.y, r.z += del * n.z, r.w += del * n.w;
 
-    if (!isfinite(TOT) || !isfinite(SCA)) TOT = 0, om = -1, n_nonfinite++;
+    if (!isfinite(TOT) || !isfinite(SCA)) {  // rare: counted straight into the launch counter
+      TOT = 0, om = -1;
+      atomicAdd(&P.ctr->nonfinite, 1ull);
+    }
 
     if (ADJ && om != -1) {
       if (om == ofla)
@@ -920,7 +918,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       live = false;
       if (ADJ) {
         if (!pass1 && nw > 0) pend = true;
-        if (pass1 && nw != 0) n_nonfinite++;  // the scoring pass lost a hit of the summing pass
+        if (pass1 && nw != 0) atomicAdd(&P.ctr->nonfinite, 1ull);  // the scoring pass lost a hit of the summing pass
       } else if (P.hit_photons && (ph.nhw & 0xffffu)) {  // start state of a photon that hit, for the adjoint
         const uint64_t em = __ballot(true);
         const int leader = __ffsll((unsigned long long)em) - 1;
@@ -1064,8 +1062,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
   // per-launch counters: one atomic per wave; steps, photons and killed photons are
   // already wave totals, the others are summed over the lanes
   constexpr int NC = COUNT ? 8 : 4;
-  unsigned long long v[8] = {n_steps, n_photons, n_killed, n_nonfinite, n_cross, n_str, n_dom, n_scat};
-  for (int q = 3; q < NC; q++) {
+  unsigned long long v[8] = {n_steps, n_photons, n_killed, 0, n_cross, n_str, n_dom, n_scat};
+  for (int q = 4; q < NC; q++) {
     unsigned long long t = v[q];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
     v[q] = t;
