@@ -125,8 +125,11 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
 // swapped back at the end.  The swaps are selects, so that lanes with different
 // largest components do not diverge (ties resolve as in the reference: y only if
 // it exceeds both others, z only if it exceeds x and is not below y).
-// NORM = false leaves out the final renormalisation, for callers whose next
-// operation is a linear map followed by a normalisation (which absorbs it).
+// NORM = false leaves out the final renormalisation.  The turn is perpendicular to
+// n and of length si, so |n'|^2 = (1 - cs)^2 |n|^2 + si^2: an error e of |n| becomes
+// (1 - cs)^2 e plus rounding, which does not grow over a photon's scatterings; the
+// renormalisation (one rsqrt and six operations) is kept for emission, where the
+// reference's direction enters once.
 template <bool NORM = true>
 __device__ __forceinline__ void rotate(float cs, float si, float4& n, Mwc& s) {
   const float x = n.x, y = n.y, z = n.z;
@@ -882,7 +885,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       else {
         ppc_hip_hit h;
         h.i = om, h.t = r.w, h.n = info[blockDim.x], h.z = ph.nhw >> 16;
-        h.pth = acosf(n.z);
+        h.pth = acosf(fminf(fmaxf(n.z, -1.f), 1.f));  // |n| is 1 up to rounding
         h.pph = atan2f(n.y, n.x);
         sqd *= P.xR - 1;
         const float4 dm = P.dom[om];
@@ -1041,8 +1044,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           n.y = n1 * P.azy + n2 * P.azx;
           n.z *= ar.w;
           normalize3(n.x, n.y, n.z);
-        } else
-          rotate(xi, fsqrt(xi * (2 - xi)), n, s);
+        } else  // the rotation keeps |n| (see rotate): no renormalisation between scatterings
+          rotate<false>(xi, fsqrt(xi * (2 - xi)), n, s);
       }
     }
     ph.r = r, ph.n = n;
