@@ -47,6 +47,21 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// element i of a table in global memory: a global (not flat) load whose base stays in
+// scalar registers and whose byte offset is one 32-bit vector register (tables here are
+// far below 4 GiB); a flat load would also hold up the next wait for LDS results
+// (read as a plain float vector: the HIP vector types have no constructor from a global
+// address-space reference)
+typedef __attribute__((address_space(1))) const char gbyte;
+template <class T>
+__device__ __forceinline__ T ldg(const T* base, uint32_t i) {
+  typedef __attribute__((ext_vector_type(sizeof(T) / 4))) float raw;
+  const raw v = *(const __attribute__((address_space(1))) raw*)((gbyte*)base + i * (uint32_t)sizeof(T));
+  T out;
+  __builtin_memcpy(&out, &v, sizeof(T));
+  return out;
+}
+
 // ---------------------------------------------------------------- RNG
 // SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014): the start state of photon i
 // when photons are seeded one by one
@@ -709,7 +724,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         const int nx = P.cm_nx, ny = P.cm_ny;
         const float gx = (r.x - x0) * rs, gy = (r.y - y0) * rs;
         const bool in = (gx >= 0.f) & (gy >= 0.f) & (gx < (float)nx) & (gy < (float)ny);
-        if (in) ph.clear = cmap[(int)gy * nx + (int)gx];
+        if (in) ph.clear = ldg(cmap, (uint32_t)((int)gy * nx + (int)gx));
       }
       if (ph.clear > reach) {
         search = false;
@@ -762,8 +777,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
                 for (int l = n1; l <= n2; l++) {
                   if (l == old) continue;
                   if (COUNT) n_dom++;
-                  const float4 dm = P.dom[l];
-                  const float F = P.uniform_dom ? 1.f : P.domF[l];
+                  const float4 dm = ldg(P.dom, (uint32_t)l);
+                  const float F = P.uniform_dom ? 1.f : ldg(P.domF, (uint32_t)l);
                   float a = 0, bb = 0, cc = 0, d2;
                   const float f = F > 0 ? sq(frcp(F)) : 0;
                   d2 = dm.x - r.x;
