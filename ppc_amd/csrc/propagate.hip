@@ -229,7 +229,10 @@ __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const f
     float wx = (nr - T.lr[i]) * rdw, wy = (T.lr[j] - nr) * rdw;
     float qx = pj.x * wx + pi.x * wy;
     float qy = pj.y * wx + pi.y * wy;
-    if (P.vthk != 0) dz = fdiv(dz, 1 - lrdz * qy);
+    if (P.vthk != 0) {
+      asm volatile("");  // a branch on the uniform flag, not a select that always divides
+      dz = fdiv(dz, 1 - lrdz * qy);
+    }
     return qx + qy * (z - k);
   } else if constexpr (TILT == 2) {
     float qx = P.cb00 * r.x + P.cb10 * r.y;
@@ -286,7 +289,10 @@ __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const f
       float2 v = T.lp[cz * T.lp_stride + k];
       ax += v.x * wz, ay += v.y * wz;
     }
-    if (P.vthk != 0) dz = fdiv(dz, 1 - P.lrdz * ay);
+    if (P.vthk != 0) {
+      asm volatile("");  // a branch on the uniform flag, not a select that always divides
+      dz = fdiv(dz, 1 - P.lrdz * ay);
+    }
     return ax + ay * (z - k);
   } else {
     return 0.f;
@@ -678,11 +684,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       sca = 0, tot = 0;
       float ais = SCA, aia = TOT;
       const float ranz = frcp(anz);
+      const int jend = sign < 0 ? 0 : P.size - 1;  // the last layer in the direction of travel
       while (true) {  // layer walk
         const float bs = w.sca(T, J), ba = w.abs(T, J);
         float dsca = fdiv(ais, bs), dtot = fdiv(aia, ba);
         float dist = fminf(dsca, dtot) * anz;
-        if ((sign < 0 ? J > 0 : J < P.size - 1) && dist > ahx) {
+        if ((J != jend) & (dist > ahx)) {
           float del = ahx * ranz;
           sca += del, tot += del;
           ais -= bs * del, aia -= ba * del;
