@@ -157,7 +157,7 @@ typedef struct {
   int32_t count_events;    /* 1: also count layer crossings, string and DOM tests, scatterings (slower) */
   int32_t record_starts;   /* 1: record the start state of every photon that hits (adjoint), max_hits of them */
   uint32_t chunk;          /* photons a wave claims per atomic on the work counter; 0: num/(128 waves), 64..1024 */
-  int32_t tilt_lds;        /* 1-d/2-d tilt table: 0 in LDS when it fits in 16 KB, 1 in LDS when it fits, -1 in HBM */
+  int32_t tilt_lds;        /* 2-d tilt table: 0 in LDS when it fits in 16 KB, 1 in LDS when it fits, -1 in HBM (the 1-d tilt table is always read from HBM) */
   int32_t no_clearance;    /* 1: search the cell grid at every step (no skipping of steps far from all strings) */
 } ppc_hip_config;
 

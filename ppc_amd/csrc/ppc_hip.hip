@@ -505,8 +505,10 @@ int ppc_hip_upload(ppc_hip_ctx* c, const ppc_hip_tables* t) {
   P.lp_stride = lp_stride;
   {
     const size_t lp_bytes = lp.size() * sizeof(float2);
+    // the 1-d tilt reads its table from global memory; the 2-d tilt's three columns per
+    // step come from LDS when the table fits the budget of the tilt_lds setting
     const int mode = c->cfg.tilt_lds;
-    P.lp_in_lds = mode < 0 ? 0 : lp_bytes <= (mode > 0 ? size_t(40960) : size_t(16384)) ? 1 : 0;
+    P.lp_in_lds = t->tmod != 2 || mode < 0 ? 0 : lp_bytes <= (mode > 0 ? size_t(40960) : size_t(16384)) ? 1 : 0;
   }
   if (P.lp_in_lds) P.off_lp = put(lp.data(), sizeof(float2) * lp.size());
   img.resize((img.size() + 15) & ~size_t(15));

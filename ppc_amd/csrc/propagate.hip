@@ -224,7 +224,10 @@ __device__ __forceinline__ float zshift(const KParams& P, const Tabs& T, const f
     if (1 <= lnum - 2) j = nr < lr1 ? 1 : j;
     const float z = (r.z - lmin) * lrdz;
     const int i = j - 1, k = min(max(rd(z), 0), lpts);
-    float2 pi = T.lp[i * T.lp_stride + k], pj = T.lp[j * T.lp_stride + k];
+    // from global memory (L1/L2-resident; measured faster than an LDS copy here, which
+    // would cost occupancy), with 32-bit offsets
+    const uint32_t e = i * T.lp_stride + k;
+    const float2 pi = ldg(P.lp, e), pj = ldg(P.lp, e + T.lp_stride);
     const float rdw = T.lr[kLMAX + j];  // 1 / (lr[j] - lr[j-1]), prepared on the host
     float wx = (nr - T.lr[i]) * rdw, wy = (T.lr[j] - nr) * rdw;
     float qx = pj.x * wx + pi.x * wy;
@@ -685,6 +688,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       float ais = SCA, aia = TOT;
       const float ranz = frcp(anz);
       const int jend = sign < 0 ? 0 : P.size - 1;  // the last layer in the direction of travel
+      float bend;  // absorption coefficient of the layer the walk ends in
       while (true) {  // layer walk
         const float bs = w.sca(T, J), ba = w.abs(T, J);
         float dsca = fdiv(ais, bs), dtot = fdiv(aia, ba);
@@ -698,6 +702,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
           if (COUNT) n_cross++;
         } else {
           sca += dsca, tot += dtot;
+          bend = ba;
           break;
         }
       }
@@ -706,7 +711,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       if (tot < sca)
         sca = tot, tot = 0;
       else
-        tot = (tot - sca) * w.abs(T, J);
+        tot = (tot - sca) * bend;
     }
 
     int om = -1;
