@@ -529,6 +529,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
 
   Photon ph;
   ph.TOT = 0, ph.SCA = 0, ph.old = -1, ph.ofla = -1;
+  ph.r = make_float4(0.f, 0.f, 0.f, 0.f), ph.n = make_float4(0.f, 0.f, 1.f, 0.f);
+  ph.clear = 0.f, ph.nhw = 0, ph.w.la = 0.f, ph.w.lk = 0.f, ph.w.abl = 0.f, ph.w.lay = 0;
   bool live = false;
   bool done = false;  // no more photons to claim (wave-uniform)
   uint32_t next_static = gid;
@@ -632,12 +634,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
         n_killed += __popcll(__ballot(start && !live));
       }
     }
-    if (!__any(live)) {
-      if (done) break;
-      continue;
-    }
+    if (done && !__any(live)) break;
     n_steps += __popcll(__ballot(live));
-    if (live) {
+    {
+    const bool alive = live;
     // -------- one transport step (gpu/pro.cu:486-889) --------
     float4 r = ph.r, n = ph.n;
     float TOT = ph.TOT, SCA = ph.SCA;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     const int ofla = ph.ofla;
     const Wave w = ph.w;
 
-    if (SCA == 0) SCA = -flog(s());  // old keeps the DOM of the previous step / the emitter
+    if (alive && SCA == 0) SCA = -flog(s());  // old keeps the DOM of the previous step / the emitter
     float sca, tot;
     const int sign = n.z < 0 ? -1 : 1;
     const float anz = sign * n.z;
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     bool search = true;
     if (P.clear_map) {
       const float reach = sca * fsqrt(np);
-      if (ph.clear <= reach) {
+      if (alive & (ph.clear <= reach)) {
         ph.clear = 0.f;
         // the map's parameters read together (one wait for their scalar loads), and the
         // bounds tested without short-circuit branches
@@ -745,6 +745,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       } else
         ph.clear = 0.f;
     }
+    search &= alive;
     if (search) {
       float rfx = r.x + sca * n.x, rfy = r.y + sca * n.y;
       float pix = P.cb00 * r.x + P.cb10 * r.y, piy = P.cb01 * r.x + P.cb11 * r.y;
@@ -874,7 +875,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     } else
       SCA = 0, TOT = tot * nr;
 
-    if (ADJ && G != 0.f) {  // path-length scores of this segment
+    if (ADJ && alive && G != 0.f) {  // path-length scores of this segment
       const float ca = -G * nr * w.lk, cb = -G * w.la;
       split_layers(P, z, I, sign, anz, edh, del, [&](int k, float l) {
         atomicAdd(&gacc[k], ca * P.adj_dpda[k] * l);
@@ -884,7 +885,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
 
     r.x += del * n.x, r.y += del * n.y, r.z += del * n.z, r.w += del * n.w;
 
-    if (!isfinite(TOT) || !isfinite(SCA)) {  // rare: counted straight into the launch counter
+    if (alive && (!isfinite(TOT) || !isfinite(SCA))) {  // rare: counted straight into the launch counter
       TOT = 0, om = -1;
       atomicAdd(&P.ctr->nonfinite, 1ull);
     }
@@ -943,10 +944,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
       }
     }
 
-    if (TOT < XXX) {
+    // (written so that NaN ends here too: an idle lane's 0/0 in a layer without absorption)
+    if (!(TOT >= XXX)) {
       TOT = 0;
       live = false;
-      if (ADJ) {
+      if (!alive) {
+      } else if (ADJ) {
         if (!pass1 && nw > 0) pend = true;
         if (pass1 && nw != 0) atomicAdd(&P.ctr->nonfinite, 1ull);  // the scoring pass lost a hit of the summing pass
       } else if (P.hit_photons && (ph.nhw & 0xffffu)) {  // start state of a photon that hit, for the adjoint
