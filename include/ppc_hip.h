@@ -148,7 +148,7 @@ typedef struct {
 typedef struct {
   int32_t blocks;          /* workgroups; default: 1792/threads per CU (28 waves per CU, 7 per SIMD) */
   int32_t threads;         /* lanes per workgroup (multiple of 64); default 256 */
-  int32_t refill;          /* refill a wave once this many of its 64 lanes are idle; default 8 */
+  int32_t refill;          /* refill a wave once this many of its 64 lanes are idle; default 10 */
   int32_t deterministic;   /* 1: photon i always runs on lane i % lanes (the reference's DTMN) */
   uint32_t max_bunches;    /* bunch buffer capacity; default 1<<20 */
   uint32_t max_hits;       /* hit buffer capacity; default 1<<24 */

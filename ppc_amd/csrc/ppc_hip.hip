@@ -256,7 +256,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   }
   // 28 waves per CU (7 per SIMD: the occupancy the kernel's register budget is compiled for)
   if (k.blocks <= 0) k.blocks = c->cus * std::max(1, ppc::kWavesPerSimd * 4 * 64 / k.threads);
-  if (k.refill <= 0) k.refill = 8;
+  if (k.refill <= 0) k.refill = 10;
   if (k.chunk > 1024) {
     delete c;
     return fail(PPC_HIP_E_ARG, "chunk %u above 1024", k.chunk);

@@ -20,7 +20,7 @@ from . import api
 DEFAULT_SPACE = {
     "threads": (256, 512),
     "waves_per_cu": (16, 28),
-    "refill": (4, 8, 16),
+    "refill": (8, 10, 16),
     "chunk": (0, 256),
     "tilt_lds": (0, -1),
 }
