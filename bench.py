@@ -54,7 +54,7 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (AMD published)
 # HBM bytes per photon of the propagation kernel on this workload, from rocprofv3 PMC runs
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy of known size gives):
 # profiles/propagate_c2_cascade_pmc.md
-HBM_BYTES_PER_PHOTON = 11.1
+HBM_BYTES_PER_PHOTON = 0.84
 
 # Algorithmic FP32 operations of the reference algorithm per event, counted from
 # gpu/pro.cu (add/mul = 1, FMA = 2; divide, sqrt, rsqrt, log, exp, sin, cos = 1).
