@@ -5,8 +5,9 @@
 // the compiler needs them again; passed by value, its ~600 B would occupy SGPRs
 // the kernel needs).  The small per-layer, per-string and cell tables are packed
 // by the host into one contiguous image that every workgroup copies into LDS at
-// start; the large tables (DOM positions, tilt tables that do not fit in LDS,
-// wavelength factors, the clearance map) stay in HBM and are reached through L2.
+// start; the large tables (DOM positions, the 1-d tilt table, a 2-d tilt table that
+// does not fit in LDS, wavelength factors, the clearance map) stay in HBM and are
+// reached through L2.
 #ifndef PPC_KPARAMS_H
 #define PPC_KPARAMS_H
 

@@ -137,7 +137,9 @@ def _crn_check(env, which, direction, eps, repeats=CRN_REPEATS):
         fd.std(ddof=1) / np.sqrt(repeats), 100 * d.mean() / abs(fd.mean()),
         100 * d.std(ddof=1) / np.sqrt(repeats) / abs(fd.mean())), flush=True)
     assert abs(lr.mean()) > 5 * err  # the comparison is informative
-    assert abs(lr.mean() - fd.mean()) <= 4 * err + 0.02 * abs(fd.mean())
+    # paired: the standard error of the photon-by-photon difference (the 5 % accuracy
+    # itself is asserted on the C5 workload, tests/test_gpu_c5.py)
+    assert abs(d.mean()) <= 4 * d.std(ddof=1) / np.sqrt(repeats) + 0.01 * abs(fd.mean())
 
 
 def test_absorption_gradient_vs_crn_differences(env):
