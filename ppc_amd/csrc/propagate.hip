@@ -5,8 +5,9 @@
 // claims a chunk of photon indices from the global counter with one atomic, and
 // when at least `refill` of its 64 lanes are idle it gives them the next indices
 // of its chunk; each idle lane emits its photon (source sampling) before the wave
-// steps again.  One loop iteration is one transport
-// step of the reference kernel (gpu/pro.cu:486-889): absorption anisotropy,
+// steps again (lanes still idle step along on a state that changes nothing).  One
+// loop iteration is one transport step of the reference kernel
+// (gpu/pro.cu:486-889): absorption anisotropy,
 // layer walk to the next scattering/absorption point, DOM search through the
 // cell grid, hit recording, birefringence, and scattering.  Hits are compacted
 // per wave with ballot + prefix count into the HBM hit buffer.
@@ -636,6 +637,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     }
     if (done && !__any(live)) break;
     n_steps += __popcll(__ballot(live));
+    // Every lane runs the step, a lane without a photon on a state that changes nothing:
+    // it draws nothing (SCA stays 0), searches and records nothing, its zero optical
+    // depths walk no layer and end in the death test (a 0/0 there too).  No per-lane
+    // region around the step and one loop back-edge: with either, the register allocator
+    // kept the photon state in two register sets and copied it between them every step.
     {
     const bool alive = live;
     // -------- one transport step (gpu/pro.cu:486-889) --------
