@@ -263,7 +263,7 @@ def batch_run(steps, seed=5):
             "photons_per_s": round(num * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 2),
             "tuned_shape": best, "candidates": len(table),
             "untuned_default_ms_at_2e8": next((round(t, 2) for sh, t in table
-                                               if sh == {"threads": 256, "waves_per_cu": 28, "refill": 8, "chunk": 0,
+                                               if sh == {"threads": 256, "waves_per_cu": 28, "refill": 10, "chunk": 0,
                                                          "tilt_lds": 0}), None),
             "tuned_ms_at_2e8": round(min(t for _, t in table), 2)}
 
