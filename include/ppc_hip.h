@@ -325,16 +325,19 @@ typedef struct {
 } ppc_hip_accept_tables;
 
 /* the fate of one raw hit: accepted by PMT pmt (>= 0) or rejected (-1); for an accepted hit
- * the photon's wavelength [nm], the impact point relative to the DOM centre [m] and the
- * photon direction (the reference's pout, gpu/f2k.cxx:340-352) */
+ * the photon's wavelength [nm], the impact point relative to the DOM centre [m], the
+ * photon direction (the reference's pout, gpu/f2k.cxx:340-352) and the time the sensor adds
+ * to the hit [ns] (the transit through a wavelength-shifting tube, pmt 0 or 1 naming its end,
+ * gpu/ini.cxx:211-243; 0 for other sensors) */
 typedef struct {
   int32_t pmt;
   float wv;
   float r[3], n[3];
+  float dt;
 } ppc_hip_accepted;
 
 /* copy the acceptance tables to the device (no counterpart: the reference applies them on
- * the host in print(), gpu/f2k.cxx:247-363; wavelength-shifting tube sensors are refused) */
+ * the host in print(), gpu/f2k.cxx:247-363) */
 int ppc_hip_upload_accept(ppc_hip_ctx* ctx, const ppc_hip_accept_tables* tables);
 /* decide acceptance, PMT and wavelength of the min(stored, cap) hits of the last launch on the
  * device, one record per hit in the order of ppc_hip_hits, with counter-based random numbers

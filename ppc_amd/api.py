@@ -44,7 +44,7 @@ class Hit(ctypes.Structure):
 HIT_DTYPE = np.dtype([("i", "<u4"), ("n", "<u4"), ("z", "<u4"), ("t", "<f4"), ("pth", "<f4"), ("pph", "<f4"),
                       ("dth", "<f4"), ("dph", "<f4")])
 RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x", "<u4"), ("c", "<u4"), ("a", "<u4")])
-ACCEPTED_DTYPE = np.dtype([("pmt", "<i4"), ("wv", "<f4"), ("r", "<f4", (3,)), ("n", "<f4", (3,))])
+ACCEPTED_DTYPE = np.dtype([("pmt", "<i4"), ("wv", "<f4"), ("r", "<f4", (3,)), ("n", "<f4", (3,)), ("dt", "<f4")])
 ADJ_RECORD_DTYPE = np.dtype([("photon", "<u4"), ("x0", "<u4"), ("c0", "<u4"), ("lane", "<u4")])
 LAYER_DTYPE = np.dtype([("s0", "<f4"), ("s1", "<f4"), ("p", "<f4"), ("t", "<f4")])
 

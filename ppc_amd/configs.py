@@ -455,8 +455,33 @@ def write_c1ng(path, nrnd=131072):
     return path
 
 
+def write_c1wom(path, nrnd=131072):
+    """The 7-string ice with wavelength-shifting tube sensors (gpu/ini.cxx:211-243, 731-761):
+    om.conf with a tube of radius 0.057 m and length 0.76 m (beta -2, Rz < 0: a cylinder, read
+    out at its two ends, "PMTs" 0 and 1 along the axis), om.map placing tubes on string 2 and
+    on string 1 except its DOM 30 (the flasher of the C1 case keeps the one-PMT DOM), the
+    tube's wavelength acceptance om.wv_3.0, and the direction mesh om.dirs."""
+    write_c1(path, nrnd=nrnd)
+    with open(os.path.join(path, "om.conf"), "w") as f:
+        f.write("wom 3 0.5 -2 0.057 -0.38 2 0.0 0.0\n180.0 0.0\n")
+    with open(os.path.join(path, "om.dirs"), "w") as f:
+        for k, d in enumerate(_fibonacci_dirs(2000)):
+            f.write("%d %.7f %.7f %.7f\n" % (k, d[0], d[1], d[2]))
+    with open(os.path.join(path, "om.map"), "w") as f:
+        for s in (1, 2):
+            for d in range(1, 61):
+                if (s, d) != (1, 30):
+                    f.write("%d %d 3\n" % (s, d))
+    with open(os.path.join(path, "om.wv_3.0"), "w") as f:
+        lo, hi = 280.0, 560.0
+        for w in np.arange(lo, hi + 1e-3, 10.0):
+            x = (w - lo) / (hi - lo)
+            f.write("%.1f %.6f\n" % (w, 50.0 * math.sin(math.pi * x) ** 2))
+    return path
+
+
 _WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9, "cbig": write_cbig,
-            "c1ng": write_c1ng}
+            "c1ng": write_c1ng, "c1wom": write_c1wom}
 
 
 def ensure(name, nrnd=131072):

@@ -1,8 +1,9 @@
 // accept.hip -- the reference's hit post-processing (gpu/f2k.cxx:247-363 with
 // itype::getPMT, gpu/ini.cxx:169-280) as a device epilogue: one lane per raw hit
-// decides the relative DOM efficiency, the PMT (angular sensitivity or the PMT
-// layout of a multi-PMT sensor), the cable shadow and the photon's wavelength
-// within its bin, and writes the accepted photon's impact point and direction.
+// decides the relative DOM efficiency, the PMT (angular sensitivity, the PMT
+// layout of a multi-PMT sensor, or the end of a wavelength-shifting tube with its
+// transit time), the cable shadow and the photon's wavelength within its bin, and
+// writes the accepted photon's impact point, direction and added time.
 //
 // Random numbers: the reference draws them from one host stream in hit order;
 // here every draw is a counter-based uniform of (seed, hit, draw) (SplitMix64
@@ -59,6 +60,42 @@ __device__ float xarea(const ppc_hip_sensor& S, float dot) {
                   : -4 * S.Rr * S.Rz * sqrtf(1 - dot * dot);
 }
 
+// Wavelength-shifting tube sensor (gpu/ini.cxx:211-243, after the reference's model of a
+// painted quartz tube read out at both ends): pz is the axial component of the hit's
+// position vector (the impact point is minus the tube radius times it).  The photon is shifted at a depth into the tube and
+// guided to one of the two ends (the return value, chosen with rnd against the attenuation
+// along the tube); dt is the transit time: the guided path at the speed of light in quartz,
+// a delay of the parameterised spread of guided paths (drawn by rejection against absorption
+// in the quartz) and the paint's exponential re-emission.  Draws 3.. of the hit; the rejection
+// loop stops after kTries pairs (a try is rejected with probability at most 0.21, so all
+// of them with probability below 1e-43).
+constexpr int kTries = 64;
+__device__ int wls_tube(const ppc_hip_sensor& S, float pz, float rnd, uint64_t key, float& dt) {
+  constexpr float cq = 0.299792458f / 1.46f;  // speed of light in quartz [m/ns]
+  constexpr float tq = 3.0f / cq;              // absorption in quartz [ns]
+  constexpr float att = 0.678f;                // attenuation length along the tube [m]
+  constexpr float zmin = 0.01f;                // lower end of the spread's parameterisation [m]
+  constexpr float tau = 1.6f;                  // re-emission time of the paint [ns]
+  constexpr float len = 0.76f;                 // tube length [m]
+  const float ea = expf(-len / att), a = len / (1 - ea);
+  float z = S.Rr * pz * (len / fabsf(2 * S.Rz));
+  const int end = rnd - 0.5f < z / (2 * a) ? 1 : 0;
+  z = end ? len / 2 - z : len / 2 + z;
+  const float d = (z / len) * (1 - z / (2 * a)) / (1 - len / (2 * a));
+  z = -att * logf(1 - d * (1 - ea));
+  if (z < zmin) z = zmin;
+  const float x = logf(z);
+  const float s = expf(((0.0034112f * x + 0.041727f) * x + 0.45550f) * x - 1.9188f);
+  const float g = expf(((0.0012658f * x + 0.017228f) * x - 0.012750f) * x + 0.20924f);
+  float t = 0;
+  for (int k = 0; k < kTries; k++) {
+    t = s * sqrtf(2 * g * powf(1 - urand(key, 4 + 2 * k), 1 / (1 - g)) - 1);
+    if (!(t > -tq * logf(urand(key, 5 + 2 * k)))) break;
+  }
+  dt = t + z / cq - tau * logf(urand(key, 3));
+  return end;
+}
+
 }  // namespace
 
 // The random numbers of a hit depend on the seed and on the hit itself (its photon, DOM and
@@ -72,7 +109,7 @@ __global__ void accept_hits(AcceptParams A, const ppc_hip_hit* __restrict__ hits
   const uint64_t key = mix64(seed ^ mix64(((uint64_t)photon[i] << 32 | h.i) ^ ((uint64_t)__float_as_uint(h.t) << 17)));
   ppc_hip_accepted o;
   o.pmt = -1, o.wv = 0;
-  o.r[0] = o.r[1] = o.r[2] = 0, o.n[0] = o.n[1] = o.n[2] = 0;
+  o.r[0] = o.r[1] = o.r[2] = 0, o.n[0] = o.n[1] = o.n[2] = 0, o.dt = 0;
   if ((int32_t)h.i < 0 || (int32_t)h.i >= A.gsize || h.z >= PPC_HIP_WNUM) {
     out[i] = o;
     return;
@@ -92,7 +129,10 @@ __global__ void accept_hits(AcceptParams A, const ppc_hip_hit* __restrict__ hits
     const ppc_hip_sensor& S = A.sensors[d.sensor];
     const float rnd = urand(key, 1);
     int pmt = -1;
-    if (S.def) {  // one PMT: angular sensitivity along the DOM axis
+    float dt = 0;
+    if (S.beta == -2)  // wavelength-shifting tube: the end that records the photon
+      pmt = wls_tube(S, rz, rnd, key, dt);
+    else if (S.def) {  // one PMT: angular sensitivity along the DOM axis
       if (S.mas > 0) {
         const float x = nx * d.tilt[0] + ny * d.tilt[1] + nz * d.tilt[2];
         float y = 1, sum = S.s[0];
@@ -137,6 +177,7 @@ __global__ void accept_hits(AcceptParams A, const ppc_hip_hit* __restrict__ hits
       o.wv = lo + (hi - lo) * urand(key, 2);
       o.r[0] = -d.R * rx, o.r[1] = -d.R * ry, o.r[2] = -d.R * rz;
       o.n[0] = nx, o.n[1] = ny, o.n[2] = nz;
+      o.dt = dt;
     }
   }
   out[i] = o;

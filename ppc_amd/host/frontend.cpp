@@ -850,7 +850,10 @@ void print() {
     const DomInfo& n = det.names[h.i];
     if (dev) {
       const ppc_hip_accepted& a = be.acc[idx];
-      if (a.pmt >= 0) output(h, n, a.pmt, a.wv, a.r, a.n);
+      if (a.pmt >= 0) {
+        h.t += a.dt;
+        output(h, n, a.pmt, a.wv, a.r, a.n);
+      }
       continue;
     }
     bool flag = n.hv > 0;

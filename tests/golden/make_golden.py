@@ -333,10 +333,11 @@ def make_cli(n=int(1.6e9)):
     print("cli c1: %d photons, %d HIT lines" % (photons, len(rows)))
 
 
-def make_cli_nextgen(n=int(8e9)):
+def make_cli_nextgen(n=int(8e9), cfg="c1ng"):
     """accepted HIT lines ("HIT str dom_pmt ...") of the reference command line on the C1 flasher
-    in the 7-string ice with nextgen multi-PMT modules on strings 1 and 2 (configs.write_c1ng)"""
-    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure("c1ng")})
+    in the 7-string ice with nextgen sensors on strings 1 and 2: multi-PMT modules
+    (configs.write_c1ng) or wavelength-shifting tubes (configs.write_c1wom)"""
+    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure(cfg)})
     procs = [subprocess.Popen([U.REF_EXE, "1", "30", "%d" % (n // 8), str(k)], env=env, stdout=subprocess.PIPE,
                               stderr=subprocess.PIPE, text=True) for k in range(8)]
     # one thread per process, so that the processes run side by side and none blocks on a full pipe
@@ -351,10 +352,10 @@ def make_cli_nextgen(n=int(8e9)):
                 dom, pmt = f[2].split("_")
                 rows.append([float(f[1]), float(dom), float(pmt), float(f[3]), float(f[4])])
     arr = np.array(rows, dtype=np.float64)
-    np.savez_compressed(os.path.join(HERE, "cli_c1ng.npz"), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
+    np.savez_compressed(os.path.join(HERE, "cli_%s.npz" % cfg), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
                         dom=arr[:, 1].astype(np.int16), pmt=arr[:, 2].astype(np.int16),
                         t=arr[:, 3].astype(np.float32), wv=arr[:, 4].astype(np.float32))
-    print("cli c1ng: %d photons, %d HIT lines" % (photons, len(rows)))
+    print("cli %s: %d photons, %d HIT lines" % (cfg, photons, len(rows)))
 
 
 def cli_f2k_lines(events=8, E=1000.0):
@@ -444,6 +445,8 @@ if __name__ == "__main__":
         make_cli()
     if "cli" in what or "cli_nextgen" in what:
         make_cli_nextgen()
+    if "cli" in what or "cli_wom" in what:
+        make_cli_nextgen(cfg="c1wom")
     if "cli" in what or "cli_f2k" in what:
         make_cli_f2k()
     if "cli" in what or "cli_f2k_muon" in what:

@@ -154,3 +154,85 @@ def test_library_device_acceptance_matches_host(c1_dir):
     pks = stats.ks_2samp(ho["wv"], de["wv"]).pvalue
     print("per-DOM chi2 p %.3f over %d DOMs, wavelength KS p %.3f" % (pchi, m.sum(), pks), flush=True)
     assert pchi > 0.01 and pks > 0.01
+
+
+def _tube_transit(end, zz, rng):
+    """the transit time through a wavelength-shifting tube of the reference's model
+    (gpu/ini.cxx:211-243) for hits at axial coordinate zz [m] read out at end `end`, sampled
+    with numpy's generator: the law the device's draws must follow"""
+    cq, att, zmin, tau, L = 0.299792458 / 1.46, 0.678, 0.01, 1.6, 0.76
+    a = L / (1 - np.exp(-L / att))
+    z = np.where(end == 1, L / 2 - zz, L / 2 + zz)
+    d = (z / L) * (1 - z / (2 * a)) / (1 - L / (2 * a))
+    z = np.maximum(-att * np.log(1 - d * (1 - np.exp(-L / att))), zmin)
+    x = np.log(z)
+    s = np.exp(((0.0034112 * x + 0.041727) * x + 0.45550) * x - 1.9188)
+    g = np.exp(((0.0012658 * x + 0.017228) * x - 0.012750) * x + 0.20924)
+    t = np.full(len(z), np.nan)
+    todo = np.ones(len(z), bool)
+    while todo.any():
+        k = np.flatnonzero(todo)
+        u1, u2 = rng.random(len(k)), rng.random(len(k))
+        tk = s[k] * np.sqrt(2 * g[k] * (1 - u1) ** (1 / (1 - g[k])) - 1)
+        acc = tk <= -(3.0 / cq) * np.log(u2)
+        t[k[acc]] = tk[acc]
+        todo[k[acc]] = False
+    return t + z / cq - tau * np.log(rng.random(len(z)))
+
+
+def test_device_tube_sensors_follow_the_tube_model():
+    """Wavelength-shifting tube sensors (beta -2, cylinders) on strings 1 and 2 of the 7-string
+    ice, C1 flasher, acceptance on the device: per accepted tube hit the end is 1 with
+    probability 1/2 + zz/(2a) (zz the impact point's axial coordinate scaled to the 0.76 m
+    tube, a = 0.76/(1 - exp(-0.76/0.678))); the sum of (end - p) zz over the hits, which a
+    swapped or missing axial term moves by tens of sigma, is within 5 sigma of 0; the transit
+    times follow the model's law given each hit's end and zz (two-sample KS against numpy
+    draws, p > 0.01); the efficiency alone decides acceptance (binomial, 5 sigma), and hits
+    on the one-PMT DOMs get no added time"""
+    from ppc_amd import api, configs
+
+    det = api.Detector(configs.ensure("c1wom"), {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+    T = AcceptTables.from_address(det.lib.xppc_c_accept_tables())
+    tube = [k for k in range(T.nsensor) if T.sensors[k].beta == -2]
+    assert len(tube) == 1
+    S = T.sensors[tube[0]]
+    dev = api.Device(0, deterministic=True)
+    dev.upload(det)
+    dev.upload_accept(det)
+    b = det.flasher_bunch(1, 30, num=200_000_000)
+    print("device acceptance: c1 flasher on the tube ice, %d photons" % b.num, flush=True)
+    dev.run([b])
+    h = dev.hits()
+    a = dev.accept(seed=777)
+    dev.close()
+    dsens = np.array([T.doms[i].sensor for i in range(T.gsize)])
+    dacc = np.array([T.doms[i].acc for i in range(T.gsize)])
+    drde = np.array([T.doms[i].rde for i in range(T.gsize)], dtype=np.float64)
+    dF = np.array([T.doms[i].F for i in range(T.gsize)], dtype=np.float64)
+    ist = dsens[h["i"]] == tube[0]
+    ok = a["pmt"] >= 0
+    assert ist.sum() > 20000
+    assert np.all(a["dt"][ok & ~ist] == 0) and np.all(a["pmt"][ok & ~ist] == 0)
+    rat = np.ctypeslib.as_array(T.rat, shape=(T.nacc * WNUM,)).reshape(T.nacc, WNUM)
+    pe = np.minimum(1.0, rat[dacc[h["i"][ist]], h["z"][ist]] * drde[h["i"][ist]])
+    want, sd = pe.sum(), np.sqrt((pe * (1 - pe)).sum())
+    print("tube hits accepted %d of %d, expected %.1f +- %.1f" % ((ok & ist).sum(), ist.sum(), want, sd), flush=True)
+    assert abs((ok & ist).sum() - want) < 5 * sd + 1
+    m = ok & ist
+    rz = dF[h["i"][m]] * np.cos(h["dth"][m].astype(np.float64))
+    L, att = 0.76, 0.678
+    A = L / (1 - np.exp(-L / att))
+    zz = S.Rr * rz * L / abs(2 * S.Rz)
+    p1 = np.clip(0.5 + zz / (2 * A), 0, 1)
+    end = a["pmt"][m]
+    Z = np.sum((end - p1) * zz) / np.sqrt(np.sum(p1 * (1 - p1) * zz * zz))
+    Zs = np.sum(((1 - end) - p1) * zz) / np.sqrt(np.sum(p1 * (1 - p1) * zz * zz))
+    print("ends: %d at 0, %d at 1; axial statistic %.2f sigma (ends swapped: %.1f)" % ((end == 0).sum(),
+                                                                                      (end == 1).sum(), Z, Zs), flush=True)
+    assert abs(Z) < 5 and abs(Zs) > 20
+    ref = _tube_transit(end, zz, np.random.default_rng(5))
+    dt = a["dt"][m].astype(np.float64)
+    pks = stats.ks_2samp(dt, ref).pvalue
+    print("transit time: mean %.3f vs %.3f ns, median %.3f vs %.3f, KS p %.3f" % (dt.mean(), ref.mean(), np.median(dt),
+                                                                                  np.median(ref), pks), flush=True)
+    assert np.all(dt > 0) and pks > 0.01

@@ -22,7 +22,8 @@ GOLD = os.path.join(HERE, "golden")
 ROOT = os.path.dirname(HERE)
 REFICE = "/root/reference/ice"
 
-ENVS = {"c1": {"WFLA": "405", "FWID": "10"}, "c2": {}, "c2x": {}, "c1ng": {"WFLA": "405", "FWID": "10"}}
+ENVS = {"c1": {"WFLA": "405", "FWID": "10"}, "c2": {}, "c2x": {}, "c1ng": {"WFLA": "405", "FWID": "10"},
+        "c1wom": {"WFLA": "405", "FWID": "10"}}
 
 
 def _child(code, *args, env=None):
@@ -65,7 +66,7 @@ for k in U.TABLE_FIELDS:
 
 
 @pytest.mark.ref
-@pytest.mark.parametrize("cfg", ["c1", "c2", "c2x", "c1ng"])
+@pytest.mark.parametrize("cfg", ["c1", "c2", "c2x", "c1ng", "c1wom"])
 def test_tables_match_reference(cfg):
     if not U.ref_available():
         pytest.skip("oracle/_ref not built (no reference tree)")
