@@ -845,6 +845,8 @@ int ppc_hip_upload_accept(ppc_hip_ctx* c, const ppc_hip_accept_tables* t) {
     return fail(PPC_HIP_E_ARG, "bad acceptance tables");
   for (int k = 0; k < t->nsensor; k++) {
     const ppc_hip_sensor& S = t->sensors[k];
+    if (S.beta == -2 && !(S.Rz != 0 && S.Rr > 0))
+      return fail(PPC_HIP_E_ARG, "sensor %d: wavelength-shifting tube of radius %g and half-length %g", k, S.Rr, S.Rz);
     if (S.def && (S.ns < 1 || S.ns > 16)) return fail(PPC_HIP_E_ARG, "sensor %d: %d coefficients", k, S.ns);
     if (!S.def && (S.npmt < 0 || S.pmt0 < 0 || S.pmt0 + S.npmt > t->npmt))
       return fail(PPC_HIP_E_ARG, "sensor %d: PMTs %d..%d of %d", k, S.pmt0, S.pmt0 + S.npmt, t->npmt);

@@ -54,6 +54,43 @@ def test_abi_struct_sizes():
     assert ctypes.sizeof(api.Hit) == 32
 
 
+_SIZES = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "ppc_hip.h"
+int main(void) {
+  printf("bunch %zu\nhit %zu\nconfig %zu\nstats %zu\nrecord %zu\nadj_record %zu\nlayer %zu\naccepted %zu\n"
+         "sensor %zu\ndom_accept %zu\naccepted_dt %zu\n",
+         sizeof(ppc_hip_bunch), sizeof(ppc_hip_hit), sizeof(ppc_hip_config), sizeof(ppc_hip_stats),
+         sizeof(ppc_hip_record), sizeof(ppc_hip_adj_record), sizeof(ppc_hip_layer), sizeof(ppc_hip_accepted),
+         sizeof(ppc_hip_sensor), sizeof(ppc_hip_dom_accept), offsetof(ppc_hip_accepted, dt));
+  return 0;
+}
+'''
+
+
+def test_python_mirrors_match_the_header():
+    """the ctypes structures and numpy record types of the Python binding have the sizes of
+    the C structures they mirror (a C program built against include/ppc_hip.h prints them)"""
+    from ppc_amd import api
+
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "sizes.c"), os.path.join(d, "sizes")
+        open(src, "w").write(_SIZES)
+        subprocess.run(["gcc", "-std=c99", "-I", INC, src, "-o", exe], check=True, capture_output=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    size = {l.split()[0]: int(l.split()[1]) for l in out.splitlines()}
+    assert size["bunch"] == ctypes.sizeof(api.Bunch) == 64
+    assert size["hit"] == ctypes.sizeof(api.Hit) == api.HIT_DTYPE.itemsize == 32
+    assert size["config"] == ctypes.sizeof(api.Config)
+    assert size["stats"] == ctypes.sizeof(api.Stats)
+    assert size["record"] == api.RECORD_DTYPE.itemsize
+    assert size["adj_record"] == api.ADJ_RECORD_DTYPE.itemsize
+    assert size["layer"] == api.LAYER_DTYPE.itemsize
+    assert size["accepted"] == api.ACCEPTED_DTYPE.itemsize
+    assert size["accepted_dt"] == api.ACCEPTED_DTYPE.fields["dt"][1]
+
+
 CONSUMER = r'''
 #include <cstddef>
 #include <cmath>

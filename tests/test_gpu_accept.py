@@ -187,8 +187,8 @@ def test_device_tube_sensors_follow_the_tube_model():
     tube, a = 0.76/(1 - exp(-0.76/0.678))); the sum of (end - p) zz over the hits, which a
     swapped or missing axial term moves by tens of sigma, is within 5 sigma of 0; the transit
     times follow the model's law given each hit's end and zz (two-sample KS against numpy
-    draws, p > 0.01); the efficiency alone decides acceptance (binomial, 5 sigma), and hits
-    on the one-PMT DOMs get no added time"""
+    draws, p > 0.01); the efficiency alone decides acceptance (binomial, 5 sigma); hits on
+    the one-PMT DOMs get no added time; a tube of zero length is refused at upload"""
     from ppc_amd import api, configs
 
     det = api.Detector(configs.ensure("c1wom"), {"WFLA": "405", "FWID": "10", "DTMN": "1"})
@@ -204,6 +204,11 @@ def test_device_tube_sensors_follow_the_tube_model():
     dev.run([b])
     h = dev.hits()
     a = dev.accept(seed=777)
+    # a tube without length is refused at upload (its transit times would be NaN)
+    Rz = S.Rz
+    S.Rz = 0.0
+    assert dev.lib.ppc_hip_upload_accept(dev.ctx, ctypes.byref(T)) != 0
+    S.Rz = Rz
     dev.close()
     dsens = np.array([T.doms[i].sensor for i in range(T.gsize)])
     dacc = np.array([T.doms[i].acc for i in range(T.gsize)])
