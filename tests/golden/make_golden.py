@@ -290,6 +290,9 @@ def make_replay(case, n=REPLAY_N, procs=REPLAY_PROCS):
     print("replay %s: %d photons, %d with hits, %d hits" % (case, len(st1), len(np.unique(hp)), len(hh)))
 
 
+GRID_LINES = 25100  # 100 columns of the `ppc _` dump
+
+
 def make_tables():
     for cfg, env in [("c1", {"WFLA": "405"}), ("c2", {}), ("c2x", {})]:
         out = subprocess.run([U.REF_EXE, "-"], env=_env(dict(env, PPCTABLESDIR=configs.ensure(cfg))),
@@ -302,6 +305,18 @@ def make_tables():
         out = subprocess.run([U.REF_EXE, "="], env=_env(dict(PPCTABLESDIR=configs.ensure(cfg), VTHK="1")), input=inp,
                              capture_output=True, text=True, check=True).stdout
         open(os.path.join(HERE, "zshift_%s.txt" % cfg), "w").write(out)
+    # `ppc - x y`: the layer table at a horizontal position, and the first lines of the
+    # `ppc _` grid dump (x = -750, y from -750 up, z innermost)
+    for cfg in ["c2", "c2x"]:
+        out = subprocess.run([U.REF_EXE, "-", "310.5", "-205.25"], env=_env(dict(PPCTABLESDIR=configs.ensure(cfg))),
+                             capture_output=True, text=True, check=True).stdout
+        open(os.path.join(HERE, "table_%s_xy.txt" % cfg), "w").write(out)
+        p = subprocess.Popen([U.REF_EXE, "_"], env=_env(dict(PPCTABLESDIR=configs.ensure(cfg))), stdout=subprocess.PIPE,
+                             stderr=subprocess.DEVNULL, text=True)
+        lines = [p.stdout.readline() for _ in range(GRID_LINES)]
+        p.kill()
+        p.wait()
+        open(os.path.join(HERE, "grid_%s.txt" % cfg), "w").write("".join(lines))
 
 
 def make_mwc():

@@ -2,8 +2,8 @@
 
 * field by field against the tables the reference's own ini::set() builds
   (oracle/_ref, where the reference tree is present);
-* the `ppc -` and `ppc =` outputs of the product command line against the
-  reference's outputs committed in tests/golden/;
+* the `ppc -` (also at a position), `ppc =` and `ppc _` outputs of the product
+  command line against the reference's outputs committed in tests/golden/;
 * the MWC multiplier search against the reference's rnd.txt.
 """
 import hashlib
@@ -157,6 +157,39 @@ def test_cli_zshift(cfg):
     got = _rows(_ppc(["="], {"PPCTABLESDIR": configs.ensure(cfg), "VTHK": "1"}, stdin=inp))
     ref = _rows(ref_text)
     np.testing.assert_allclose(got, ref, rtol=1e-5, atol=2e-4)
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c2x"])
+def test_cli_table_dump_at_position(cfg):
+    """`ppc - x y`: the same table with the depth shift taken at a horizontal position"""
+    from ppc_amd import configs
+
+    got = _rows(_ppc(["-", "310.5", "-205.25"], {"PPCTABLESDIR": configs.ensure(cfg)}))
+    ref = _rows(open(os.path.join(GOLD, "table_%s_xy.txt" % cfg)).read())
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c2x"])
+def test_cli_zshift_grid(cfg):
+    """`ppc _`: the first 25100 lines of the depth-shift grid dump (z, x, y, shift)"""
+    from ppc_amd import _native, configs
+
+    ref = _rows(open(os.path.join(GOLD, "grid_%s.txt" % cfg)).read())
+    e = dict(os.environ)
+    for k in ("WFLA", "FWID", "VTHK"):
+        e.pop(k, None)
+    e.update({"PPCTABLESDIR": configs.ensure(cfg), "DTMN": "1"})
+    p = subprocess.Popen([_native.ppc_exe(), "_"], env=e, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    try:
+        lines = [p.stdout.readline() for _ in range(len(ref))]
+    finally:
+        p.kill()
+        p.wait()
+    got = _rows("".join(lines))
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=0, atol=0)
+    np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=1e-5, atol=2e-4)
 
 
 def test_zshift_1d_by_hand():
