@@ -53,6 +53,34 @@ def _compare(photons, strs, doms, t, wv, fixture="cli_c1.npz"):
     assert p_rate > 0.01 and p_dom > 0.01 and p_t > 0.01 and p_w > 0.01
 
 
+def test_cli_flasher_iterations():
+    """`ppc 1 30 N*4 0`: four flashes of N photons (gpu/f2k.cxx:768-783, ppc.cu:584-588),
+    each closed by an empty segment line after its HIT lines; the four flashes have
+    Poisson-compatible hit counts and the same HIT rate as the reference CLI's single flash"""
+    from ppc_amd import _native
+
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "4e9*4", "0"], env=_env(), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    groups, cur = [], 0
+    for l in r.stdout.splitlines():
+        if l.startswith("HIT "):
+            cur += 1
+        else:
+            assert l == "", l
+            groups.append(cur)
+            cur = 0
+    assert cur == 0 and len(groups) == 4, (groups, cur)
+    photons = sum(int(m) for m in re.findall(r"photons: (\d+)", r.stderr))
+    g = np.array(groups, dtype=float)
+    chi2 = np.sum((g - g.mean()) ** 2 / g.mean())
+    gold = np.load(os.path.join(HERE, "golden", "cli_c1.npz"))
+    Nr, nr = int(gold["photons"]), len(gold["t"])
+    z = (g.sum() / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + g.sum() / photons ** 2)
+    print("hits per flash %s, chi2 %.1f/3, rate z %.2f" % (groups, chi2, z), flush=True)
+    assert g.min() > 1000 and stats.chi2.sf(chi2, 3) > 0.01 and 2 * stats.norm.sf(abs(z)) > 0.01
+
+
 def test_cli_flasher_matches_reference_cli():
     from ppc_amd import _native
 
