@@ -108,6 +108,62 @@ def test_tables_match_reference_under_switches(ice, env):
             assert float(line.split()[1]) < 1e-6
 
 
+def _overrides(case, tmp):
+    """(tables directory, environment) of a directory-override case (gpu/ini.cxx:595-625,
+    1355-1363): the ice model, tilt model, nextgen sensors, hole ice or spectrum file taken
+    from somewhere other than the tables directory"""
+    from ppc_amd import configs
+
+    if case == "ICEMODELDIR":  # the 86-string ice's layers and tilt under the 7-string geometry
+        return configs.ensure("c1"), {"ICEMODELDIR": configs.ensure("c2"), "WFLA": "405"}
+    if case == "TILTMODELDIR":  # the 2-d tilt map with the 1-d tilt ice's layers
+        return configs.ensure("c2"), {"TILTMODELDIR": configs.ensure("c2x")}
+    if case == "NEXTGENDIR":
+        return configs.ensure("c1"), {"NEXTGENDIR": configs.ensure("c1ng"), "WFLA": "405"}
+    if case == "PPCHOLEICE":
+        return configs.ensure("c1"), {"PPCHOLEICE": os.path.join(configs.ensure("c2x"), "as.dat"), "WFLA": "405"}
+    if case == "FLWL":  # another emission spectrum file, wv.<FLWL>, in the tables directory
+        import shutil
+
+        d = os.path.join(tmp, "c2flwl")
+        shutil.copytree(configs.ensure("c2"), d)
+        w = np.loadtxt(os.path.join(d, "wv.dat"))
+        w[:, 1] = w[:, 1] ** 2 / w[-1, 1]  # a steeper cumulative spectrum
+        np.savetxt(os.path.join(d, "wv.alt"), w, fmt="%.6f")
+        return d, {"FLWL": "alt"}
+    raise KeyError(case)
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("case", ["ICEMODELDIR", "TILTMODELDIR", "NEXTGENDIR", "PPCHOLEICE", "FLWL"])
+def test_tables_match_reference_under_directory_overrides(case, tmp_path):
+    """the field-by-field comparison with parts of the configuration read from elsewhere"""
+    if not U.ref_available():
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    path, env = _overrides(case, str(tmp_path))
+    out = _child(_CMP, path, HERE, env=env)
+    assert "DIFF" not in out, out
+    for line in out.splitlines():
+        if line.split()[0].endswith("-maxrel"):
+            assert float(line.split()[1]) < 1e-6
+    # the override took effect: the product's tables differ from those of the plain directory
+    plain = dict(env)
+    plain.pop(case)
+    assert _child(_HASH, path, env=env) != _child(_HASH, path, env=plain)
+
+
+_HASH = r'''
+import sys, hashlib
+import oracle_util as U
+from ppc_amd import api
+P = U.product_tables(api.Detector(sys.argv[1]))
+h = hashlib.sha256()
+for k in U.TABLE_FIELDS:
+    h.update(bytes(P.field(k)))
+print("TABLES", h.hexdigest())
+'''
+
+
 @pytest.mark.ref
 @pytest.mark.parametrize("ice", ["spice_bfr-v2", "spice_ftp-v3m"])
 def test_tables_match_reference_real_ice(ice):
