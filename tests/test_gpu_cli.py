@@ -81,6 +81,44 @@ def test_cli_flasher_iterations():
     assert g.min() > 1000 and stats.chi2.sf(chi2, 3) > 0.01 and 2 * stats.norm.sf(abs(z)) > 0.01
 
 
+def test_cli_small_launches_and_hit_quota():
+    """NPHO=8 (photons per lane per launch, gpu/ppc.cu:117-131): the flashes of `ppc 1 30 N*3 0`
+    go out in tens of launches, each printed while the next one runs, and keep the reference's
+    HIT rate and the grouping by flash; with HQUO (photons per hit slot, gpu/ppc.cu:41-43) set so
+    that a launch's hits overflow the buffer, every launch reports the overflow and prints the
+    hits that fit"""
+    from ppc_amd import _native
+
+    env = _env()
+    env["NPHO"] = "8"
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "4e9*3", "0"], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    launches = re.findall(r"photons: (\d+)  hits: (\d+)", r.stderr)
+    photons = sum(int(p) for p, _ in launches)
+    lines = r.stdout.splitlines()
+    nh = sum(l.startswith("HIT ") for l in lines)
+    assert len(launches) > 30 and lines.count("") == 3 and nh + 3 == len(lines)
+    gold = np.load(os.path.join(HERE, "golden", "cli_c1.npz"))
+    Nr, nr = int(gold["photons"]), len(gold["t"])
+    z = (nh / photons - nr / Nr) / math.sqrt(nr / Nr ** 2 + nh / photons ** 2)
+    print("NPHO=8: %d launches, %d photons, %d HIT lines, rate z %.2f" % (len(launches), photons, nh, z), flush=True)
+    assert 2 * stats.norm.sf(abs(z)) > 0.01
+
+    env["HQUO"] = "20000"
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "4e9", "0"], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    over = re.findall(r"Error: data buffer overflow occurred: (\d+)>(\d+)!", r.stderr)
+    launches = re.findall(r"photons: (\d+)  hits: (\d+)", r.stderr)
+    nh = sum(l.startswith("HIT ") for l in r.stdout.splitlines())
+    cap = int(over[0][1]) if over else 0
+    print("HQUO=20000: %d launches, %d overflowing (buffer %d hits), %d HIT lines" % (len(launches), len(over), cap,
+                                                                                 nh), flush=True)
+    assert len(over) >= len(launches) - 1 > 10
+    assert all(int(h) <= cap for _, h in launches) and 0 < nh <= cap * len(launches)
+
+
 def test_cli_flasher_matches_reference_cli():
     from ppc_amd import _native
 
