@@ -26,7 +26,7 @@ T = U.ref_tables(sys.argv[1])
 D = api.Detector(sys.argv[1])
 bad = 0
 rng = np.random.default_rng(1)
-for ptype in [0, 1, 11, 12, 13, 14]:
+for ptype in [0, 1, 10, 11, 12, 13, 14, 101, 102, 103, 104, 105, 106, 107]:
     for k in range(20):
         x, y, z, t = rng.uniform(-300, 300, 4)
         E = float(np.exp(rng.uniform(0, 12)))
