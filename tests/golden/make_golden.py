@@ -389,6 +389,21 @@ def cli_f2k_muon_lines(events=128, E=1000.0, L=1000.0):
             for _ in range(events)]
 
 
+def cli_f2k_mixed_lines():
+    """f2k TR lines of the other particle names the command line reads (gpu/f2k.cxx:555-583):
+    a hadronic cascade (hadr, munu: yield with its gaussian draw), an electron and a positron
+    (e-, e+: their longitudinal profiles), a bremsstrahlung cascade and a point flash of
+    photons (ph: addf, isotropic, E photons), around the benchmark cascade's position"""
+    cas = configs.c2_cascade()
+    x, y, z = cas["x"], cas["y"], cas["z"]
+    return ["TR 1 0 hadr %.3f %.3f %.3f 40.0 30.0 0 800 0\n" % (x, y, z),
+            "TR 1 0 e- %.3f %.3f %.3f 120.0 200.0 0 500 3\n" % (x + 20, y, z + 10),
+            "TR 1 0 e+ %.3f %.3f %.3f 80.0 300.0 0 500 5\n" % (x - 15, y + 10, z - 20),
+            "TR 1 0 munu %.3f %.3f %.3f 150.0 90.0 0 300 1\n" % (x, y - 25, z + 30),
+            "TR 1 0 brems %.3f %.3f %.3f 60.0 10.0 0 400 2\n" % (x + 5, y + 5, z),
+            "TR 1 0 ph %.3f %.3f %.3f 0 0 0 4000000 0\n" % (x + 30, y - 10, z - 40)]
+
+
 def make_cli_f2k(name="cli_f2k_c2.npz", lines=None):
     """accepted HIT lines of the reference command line in its f2k mode (`ppc dev < f2k`),
     one process per event, on the 86-string ice"""
@@ -464,6 +479,9 @@ if __name__ == "__main__":
         make_cli_nextgen(cfg="c1wom")
     if "cli" in what or "cli_f2k" in what:
         make_cli_f2k()
+    if "cli" in what or "cli_f2k_mixed" in what:
+        # 8 processes, each reading the six TR lines (every HIT line kept, with its TR line)
+        make_cli_f2k_events("cli_f2k_mixed_c2.npz", cli_f2k_mixed_lines(), 8, sample=10 ** 9)
     if "cli" in what or "cli_f2k_muon" in what:
         # 16 processes, each reading a stream of 16 muon TR lines
         make_cli_f2k_events("cli_f2k_muon_c2.npz", cli_f2k_muon_lines(events=16), 16)

@@ -298,6 +298,50 @@ def test_cli_f2k_muon_stream_matches_reference_cli():
     assert p_rate > 0.01 and p_dom > 0.01 and p_t > 0.01 and p_w > 0.01
 
 
+def test_cli_f2k_mixed_particles_match_reference_cli():
+    """`ppc 0 < events.f2k` with the other particle names of the f2k reader (gpu/f2k.cxx:555-583)
+    on the 86-string ice: a hadronic cascade (hadr, yield with its gaussian draw), an electron,
+    a positron, a munu hadronic cascade, a bremsstrahlung cascade and a point flash of 4e6
+    photons (ph, addf: isotropic), against the reference command line on the same six TR lines
+    (tests/golden/cli_f2k_mixed_c2.npz, 8 processes); 16 repeats here, in one process.  The
+    hadronic yields vary from event to event (12 % rms for hadr), so the tests treat events as
+    the units: per particle, hits per event (Welch t-test), arrival times and wavelengths (KS:
+    within a particle the shapes do not depend on its yield, while the particles' shares of the
+    pooled hits do, and the wavelength of a hit depends on its distance to the source); per-DOM
+    counts (chi-square, p-value from permutations of whole events)"""
+    from ppc_amd import api, configs
+
+    six = G.cli_f2k_mixed_lines()
+    lines = six * 16
+    print("f2k stream: 16 x %d mixed TR lines on c2" % len(six), flush=True)
+    photons, arr, ev = _run_f2k(lines)
+    gold = np.load(os.path.join(HERE, "golden", "cli_f2k_mixed_c2.npz"))
+    gev = gold["ev"].astype(np.int64)
+    names = [l.split()[3] for l in six]
+    cp = np.bincount(ev, minlength=len(lines)).reshape(16, 6)
+    cg = np.bincount(gev, minlength=int(gold["events"])).reshape(-1, 6)
+    ps = []
+    for k, name in enumerate(names):
+        p = stats.ttest_ind(cp[:, k], cg[:, k], equal_var=False).pvalue
+        tk = stats.ks_2samp(arr[ev % 6 == k, 2], gold["t"][gev % 6 == k]).pvalue
+        wk = stats.ks_2samp(arr[ev % 6 == k, 3], gold["wv"][gev % 6 == k]).pvalue
+        ps.append((p, tk, wk))
+        print("%-5s hits per event %.1f +- %.1f vs %.1f +- %.1f (p=%.3f), KS time p=%.3f, wavelength p=%.3f"
+              % (name, cp[:, k].mean(), cp[:, k].std(ddof=1), cg[:, k].mean(), cg[:, k].std(ddof=1), p, tk, wk),
+              flush=True)
+    det = api.Detector(configs.ensure("c2"), {"DTMN": "1"})
+    key = np.asarray(det.array("str", np.int32), np.int64) * 100 + det.array("dom", np.int32)
+    pos = {int(k): i for i, k in enumerate(key)}
+    index = lambda s, d: np.array([pos[int(a) * 100 + int(b)] for a, b in zip(s, d)], dtype=np.int64)
+    E1 = _event_counts(ev // 6, arr[:, 0], arr[:, 1], 16, len(key), index)
+    E2 = _event_counts(gev // 6, gold["str"], gold["dom"], len(cg), len(key), index)
+    chi2, dof, p_dom = _permutation_chi2(E1, E2)
+    print("photons %d vs %d; DOM chi2 %.1f/%d (permutation p=%.3f)" % (photons, int(gold["photons"]), chi2, dof, p_dom),
+          flush=True)
+    # six particles, three tests each: each at p > 0.01/18
+    assert min(min(p) for p in ps) > 0.01 / 18 and p_dom > 0.01
+
+
 _LIB = r'''
 import sys, json
 import numpy as np
