@@ -119,6 +119,21 @@ def test_cli_small_launches_and_hit_quota():
     assert all(int(h) <= cap for _, h in launches) and 0 < nh <= cap * len(launches)
 
 
+@pytest.mark.parametrize("var", ["NPHO", "NPHO_0"])
+def test_cli_npho_zero_drops_the_device(var):
+    """NPHO=0 (or NPHO_<device>=0) takes the device out of use, and with no device left the
+    command line stops with the reference's message and exit code 5 (gpu/ppc.cu:120-131)"""
+    from ppc_amd import _native
+
+    env = _env()
+    env[var] = "0"
+    r = subprocess.run([_native.ppc_exe(), "1", "30", "1e6", "0"], env=env, capture_output=True, text=True,
+                       timeout=120)
+    print("%s=0: exit %d" % (var, r.returncode), flush=True)
+    assert r.returncode == 5 and "Not using device # 0!" in r.stderr and "No active GPU(s) selected!" in r.stderr
+    assert "HIT" not in r.stdout
+
+
 def test_cli_flasher_matches_reference_cli():
     from ppc_amd import _native
 
