@@ -235,7 +235,9 @@ int ppc_hip_set_states(ppc_hip_ctx* ctx, const uint64_t* states, int32_t lanes);
 /* start propagating num photons of nb bunches (kernel_c + kernel_f,
  * gpu/ppc.cu:285-306); returns without waiting for the kernel */
 int ppc_hip_launch(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num);
-/* wait for the launch and read its counters (kernel_i, gpu/ppc.cu:255-277) */
+/* wait for the launch and read its counters (kernel_i, gpu/ppc.cu:255-277); PPC_HIP_E_DEVICE
+ * when a wave of the launch did not reach the kernel's end or a photon was not started
+ * (the reference's check that every multiprocessor ran, gpu/ppc.cu:266) */
 int ppc_hip_wait(ppc_hip_ctx* ctx, ppc_hip_stats* stats);
 /* copy min(stored, cap) hits of the last launch to the host (gpu/ppc.cu:278-282);
  * photon (optional) receives the photon index of each hit */

@@ -32,7 +32,8 @@ struct Counters {
   unsigned int next;   // next photon to hand out (dynamic refill)
   unsigned int hidx;   // next free hit slot
   unsigned int nphot;  // photons with hits recorded for the adjoint
-  unsigned int pad[3];
+  unsigned int waves;  // waves that reached the end of the kernel: all of the launch's
+  unsigned int pad[2];
 };
 
 struct KParams {

@@ -1112,6 +1112,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
                                   &P.ctr->crossings, &P.ctr->strings, &P.ctr->doms, &P.ctr->scatters};
     for (int q = 0; q < NC; q++)
       if (v[q]) atomicAdd(dst[q], v[q]);
+    atomicAdd(&P.ctr->waves, 1u);  // checked on the host: every wave of the launch ran to here
   }
 }
 
