@@ -26,7 +26,9 @@ enum ppc_hip_status {
   PPC_HIP_E_DEVICE = -2,   /* HIP runtime error (the reference exits with code 2) */
   PPC_HIP_E_NOMEM = -3,    /* device memory exhausted */
   PPC_HIP_E_STATE = -4,    /* call out of order (e.g. launch before upload) */
-  PPC_HIP_E_CAPACITY = -5  /* a size exceeds the configured capacity */
+  PPC_HIP_E_CAPACITY = -5, /* a size exceeds the configured capacity */
+  PPC_HIP_E_INCOMPLETE = -6 /* a launch ended with waves that did not finish or photons not
+                               started (the reference exits with code 4, gpu/ppc.cu:266) */
 };
 
 /* photon bunch: the reference's `photon` (gpu/ini.cxx:399-419), 64 bytes */
@@ -235,7 +237,7 @@ int ppc_hip_set_states(ppc_hip_ctx* ctx, const uint64_t* states, int32_t lanes);
 /* start propagating num photons of nb bunches (kernel_c + kernel_f,
  * gpu/ppc.cu:285-306); returns without waiting for the kernel */
 int ppc_hip_launch(ppc_hip_ctx* ctx, const ppc_hip_bunch* bunches, uint32_t nb, uint32_t num);
-/* wait for the launch and read its counters (kernel_i, gpu/ppc.cu:255-277); PPC_HIP_E_DEVICE
+/* wait for the launch and read its counters (kernel_i, gpu/ppc.cu:255-277); PPC_HIP_E_INCOMPLETE
  * when a wave of the launch did not reach the kernel's end or a photon was not started
  * (the reference's check that every multiprocessor ran, gpu/ppc.cu:266) */
 int ppc_hip_wait(ppc_hip_ctx* ctx, ppc_hip_stats* stats);

@@ -745,7 +745,7 @@ int adjoint_run(ppc_hip_ctx* c, const AdjBuffers& B, const ppc_hip_bunch* bunche
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(grad, B.grad, sizeof(double) * 2 * size, hipMemcpyDeviceToHost));
   if (nrec && c->h_ctr_adj->waves != (uint32_t)(c->lanes / 64))
-    return fail(PPC_HIP_E_DEVICE, "the adjoint launch ended in %u of its %u waves", c->h_ctr_adj->waves,
+    return fail(PPC_HIP_E_INCOMPLETE, "the adjoint launch ended in %u of its %u waves", c->h_ctr_adj->waves,
                 (uint32_t)(c->lanes / 64));
   if (st) {
     float ms = 0;
@@ -969,9 +969,9 @@ int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {
      // a launch of no photons runs no kernel
     const ppc::Counters& k = *c->h_ctr;
     const uint32_t waves = c->last_num > 0 ? (uint32_t)(c->lanes / 64) : 0u;
-    if (k.waves != waves) return fail(PPC_HIP_E_DEVICE, "the launch ended in %u of its %u waves", k.waves, waves);
+    if (k.waves != waves) return fail(PPC_HIP_E_INCOMPLETE, "the launch ended in %u of its %u waves", k.waves, waves);
     if (k.photons != c->last_num)
-      return fail(PPC_HIP_E_DEVICE, "the launch started %llu of its %u photons", k.photons, c->last_num);
+      return fail(PPC_HIP_E_INCOMPLETE, "the launch started %llu of its %u photons", k.photons, c->last_num);
   }
   if (st) {
     const ppc::Counters& k = *c->h_ctr;

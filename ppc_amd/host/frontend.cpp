@@ -80,6 +80,10 @@ unsigned int pn = 0, pk = 0;
 ppc_hip_bunch p{}, pfl{};
 
 void check(int rc, const char* what) {
+  if (rc == PPC_HIP_E_INCOMPLETE) {  // gpu/ppc.cu:266
+    cerr << "Error: " << ppc_hip_last_error() << endl;
+    exit(4);
+  }
   if (rc != PPC_HIP_OK) {
     cerr << "HIP Error (" << what << "): " << ppc_hip_last_error() << endl;
     exit(2);
