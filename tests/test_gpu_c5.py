@@ -64,7 +64,7 @@ def _c5(c1_dir):
         b.num = PER_FLASHER
     sim.num = PER_FLASHER * EVENTS
     print("C5: %d flashers, %d layers; llf %.1f over %d DOM bins with data; %d repeats of %d photons"
-          % (EVENTS, size, float(llh), int((k > 0).sum()), REPEATS, sim.num), flush=True)
+          % (EVENTS, size, float(llh.detach()), int((k > 0).sum()), REPEATS, sim.num), flush=True)
     res = {(wn, x): ([], []) for wn in weights for x in "ab"}
     for r in range(REPEATS):
         sim.dev.seed_photons(1000 + r)
