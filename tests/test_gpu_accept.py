@@ -241,3 +241,73 @@ def test_device_tube_sensors_follow_the_tube_model():
     print("transit time: mean %.3f vs %.3f ns, median %.3f vs %.3f, KS p %.3f" % (dt.mean(), ref.mean(), np.median(dt),
                                                                                   np.median(ref), pks), flush=True)
     assert np.all(dt > 0) and pks > 0.01
+
+
+def _curve(beta, x):
+    """a multi-PMT sensor's PMT acceptance against the cosine x (gpu/ini.cxx:169-188)"""
+    x = np.clip(np.asarray(x, np.float64), -1, 1)
+    if beta < -1:
+        return np.sqrt(1 - x * x)
+    s = np.maximum(x, 0)
+    if beta < 1:
+        y = np.sqrt(1 - x * x) / beta
+        c = 1 - beta * beta
+        big = y > 1
+        yi = np.where(big, 1 / np.where(big, y, 1), 0.5)
+        aS = lambda v: np.arccos(v) - np.sin(2 * np.arccos(v)) / 2
+        s = s + np.where(big, (aS(yi) / c - aS(yi * np.abs(x) / np.sqrt(c)) * np.abs(x)) / np.pi, 0)
+    return s
+
+
+def test_device_multipmt_choice_follows_the_pmt_layout():
+    """Multi-PMT modules (a 12-PMT and a 2-PMT module on strings 1 and 2 of the 7-string ice),
+    C1 flasher, acceptance on the device: per PMT, the number of accepted hits against the
+    sum over the hits of efficiency x the PMT's share of the module's acceptance for the
+    photon's direction (gpu/ini.cxx:169-192, 258-277: the first PMT whose cumulative share
+    exceeds the draw), chi-square p > 0.01 over the PMTs"""
+    from ppc_amd import api, configs
+
+    det = api.Detector(configs.ensure("c1ng"), {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+    T = AcceptTables.from_address(det.lib.xppc_c_accept_tables())
+    multi = [k for k in range(T.nsensor) if T.sensors[k].def_ == 0 and T.sensors[k].beta != -2]
+    assert len(multi) == 2
+    dev = api.Device(0, deterministic=True)
+    dev.upload(det)
+    dev.upload_accept(det)
+    b = det.flasher_bunch(1, 30, num=200_000_000)
+    print("device acceptance: c1 flasher on the multi-PMT ice, %d photons" % b.num, flush=True)
+    dev.run([b])
+    h = dev.hits()
+    a = dev.accept(seed=4242)
+    dev.close()
+    dsens = np.array([T.doms[i].sensor for i in range(T.gsize)])
+    dacc = np.array([T.doms[i].acc for i in range(T.gsize)])
+    drde = np.array([T.doms[i].rde for i in range(T.gsize)], dtype=np.float64)
+    assert all(T.doms[i].azi < 0 for i in range(T.gsize))  # no cable in this ice
+    rat = np.ctypeslib.as_array(T.rat, shape=(T.nacc * WNUM,)).reshape(T.nacc, WNUM)
+    dirs = np.ctypeslib.as_array(T.pmt_dirs, shape=(T.npmt * 3,)).reshape(T.npmt, 3).astype(np.float64)
+    pth, pph = h["pth"].astype(np.float64), h["pph"].astype(np.float64)
+    n = np.stack([np.sin(pth) * np.cos(pph), np.sin(pth) * np.sin(pph), np.cos(pth)], 1)
+    obs, exp = [], []
+    for k in multi:
+        S = T.sensors[k]
+        m = dsens[h["i"]] == k
+        pe = np.minimum(1.0, rat[dacc[h["i"][m]], h["z"][m]] * drde[h["i"][m]])
+        nz = n[m, 2]
+        xa = np.pi * S.Rr * np.sqrt(S.Rz ** 2 - nz * nz * (S.Rz ** 2 - S.Rr ** 2)) if S.Rz > 0 else \
+            -4 * S.Rr * S.Rz * np.sqrt(1 - nz * nz)
+        crx = S.fx * xa
+        share = np.stack([_curve(S.beta, -(n[m] @ dirs[S.pmt0 + j])) / crx for j in range(S.npmt)], 1)
+        cum = np.minimum(np.cumsum(share, axis=1), 1.0)
+        p = np.diff(np.concatenate([np.zeros((len(cum), 1)), cum], 1), axis=1)
+        exp.append((pe[:, None] * p).sum(axis=0))
+        obs.append(np.array([np.sum(a["pmt"][m] == j) for j in range(S.npmt)], dtype=np.float64))
+        print("sensor %d (%d PMTs): %d hits, accepted %d, expected %.1f; per PMT %s vs %s" % (
+            k, S.npmt, m.sum(), obs[-1].sum(), exp[-1].sum(), obs[-1].astype(int).tolist(),
+            np.round(exp[-1], 1).tolist()), flush=True)
+    o, e = np.concatenate(obs), np.concatenate(exp)
+    assert e.min() > 20
+    chi2 = float(np.sum((o - e) ** 2 / e))
+    p = stats.chi2.sf(chi2, len(o))
+    print("chi2 %.1f / %d, p = %.3f" % (chi2, len(o), p), flush=True)
+    assert p > 0.01
