@@ -327,25 +327,22 @@ def make_mwc():
               open(os.path.join(HERE, "mwc_rnd.json"), "w"), indent=1)
 
 
-def make_cli(n=int(1.6e9)):
-    """accepted HIT lines of the reference command line on the C1 flasher"""
-    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure("c1")})
-    outs = []
-    procs = []
-    for k in range(8):
-        procs.append(subprocess.Popen([U.REF_EXE, "1", "30", "%d" % (n // 8), str(k)], env=env, stdout=subprocess.PIPE,
-                                      stderr=subprocess.PIPE, text=True))
-    photons = 0
-    for p in procs:
-        out, err = p.communicate()
-        outs.append(out)
-        photons += sum(int(l.split()[1]) for l in err.splitlines() if l.startswith("photons:"))
-    rows = [l.split() for o in outs for l in o.splitlines() if l.startswith("HIT")]
+def make_cli(n=int(1.6e9), cfg="c1", flasher=(1, 30)):
+    """accepted HIT lines of the reference command line on a flasher: the C1 flasher, or a
+    flasher of the 86-string ice with every optional feature and cable positions (c2x)"""
+    env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure(cfg)})
+    procs = [subprocess.Popen([U.REF_EXE, str(flasher[0]), str(flasher[1]), "%d" % (n // 8), str(k)], env=env,
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for k in range(8)]
+    # one thread per process, so that the processes run side by side and none blocks on a full pipe
+    with ThreadPoolExecutor(len(procs)) as ex:
+        res = list(ex.map(lambda p: p.communicate(), procs))
+    photons = sum(int(l.split()[1]) for _, err in res for l in err.splitlines() if l.startswith("photons:"))
+    rows = [l.split() for out, _ in res for l in out.splitlines() if l.startswith("HIT")]
     arr = np.array([[float(v) for v in r[1:]] for r in rows], dtype=np.float64)
-    np.savez_compressed(os.path.join(HERE, "cli_c1.npz"), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
+    np.savez_compressed(os.path.join(HERE, "cli_%s.npz" % cfg), photons=np.int64(photons), str=arr[:, 0].astype(np.int16),
                         dom=arr[:, 1].astype(np.int16), t=arr[:, 2].astype(np.float32),
                         wv=arr[:, 3].astype(np.float32))
-    print("cli c1: %d photons, %d HIT lines" % (photons, len(rows)))
+    print("cli %s: %d photons, %d HIT lines" % (cfg, photons, len(rows)))
 
 
 def make_cli_nextgen(n=int(8e9), cfg="c1ng"):
@@ -475,6 +472,8 @@ if __name__ == "__main__":
         make_cli()
     if "cli" in what or "cli_nextgen" in what:
         make_cli_nextgen()
+    if "cli" in what or "cli_c2x" in what:
+        make_cli(n=int(1.6e8), cfg="c2x", flasher=(36, 30))
     if "cli" in what or "cli_wom" in what:
         make_cli_nextgen(cfg="c1wom")
     if "cli" in what or "cli_f2k" in what:

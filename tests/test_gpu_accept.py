@@ -311,3 +311,73 @@ def test_device_multipmt_choice_follows_the_pmt_layout():
     p = stats.chi2.sf(chi2, len(o))
     print("chi2 %.1f / %d, p = %.3f" % (chi2, len(o), p), flush=True)
     assert p > 0.01
+
+
+def test_device_cable_shadow_on_full_feature_ice():
+    """Cable positions (dx.dat of the 86-string ice with every optional feature), a flasher on
+    string 36, acceptance on the device: a hit whose photon passes the cable before reaching
+    the DOM (gpu/f2k.cxx:301-322: the line back from the impact point meets the cable's
+    circle, radius 0.023 m beside the DOM at its azimuth) is never accepted, and the other
+    hits are accepted at efficiency x angular sensitivity (binomial, 5 sigma)"""
+    from ppc_amd import api, configs
+
+    det = api.Detector(configs.ensure("c2x"), {"WFLA": "405", "FWID": "10", "DTMN": "1"})
+    T = AcceptTables.from_address(det.lib.xppc_c_accept_tables())
+    dev = api.Device(0, deterministic=True)
+    dev.upload(det)
+    dev.upload_accept(det)
+    b = det.flasher_bunch(36, 30, num=60_000_000)
+    print("device acceptance: flasher 36-30 on the full-feature ice, %d photons" % b.num, flush=True)
+    dev.run([b])
+    h = dev.hits()
+    a = dev.accept(seed=99)
+    dev.close()
+    f64 = np.float64
+    D = [T.doms[i] for i in range(T.gsize)]
+    sens, acc = np.array([d.sensor for d in D]), np.array([d.acc for d in D])
+    rde, azi = np.array([d.rde for d in D], f64), np.array([d.azi for d in D], f64)
+    R, F = np.array([d.R for d in D], f64), np.array([d.F for d in D], f64)
+    tilt = np.array([list(d.tilt) for d in D], f64)
+    i = h["i"].astype(np.int64)
+    pth, pph, dth, dph = (h[k].astype(f64) for k in ("pth", "pph", "dth", "dph"))
+    nx, ny, nz = np.sin(pth) * np.cos(pph), np.sin(pth) * np.sin(pph), np.cos(pth)
+    dts = np.where(F[i] < 0, 1.0, np.sin(dth))
+    rx, ry = dts * np.cos(dph), dts * np.sin(dph)
+    # the cable's shadow, where the DOM has a cable azimuth
+    omr, rc = R[i], 0.023
+    dc = omr + rc
+    drx = -omr * rx - dc * np.cos(np.radians(azi[i]))
+    dry = -omr * ry - dc * np.sin(np.radians(azi[i]))
+    qa = nx * nx + ny * ny
+    qb = -2 * (nx * drx + ny * dry)
+    qc = drx * drx + dry * dry - rc * rc
+    disc = qb * qb - 4 * qa * qc
+    with np.errstate(invalid="ignore", divide="ignore"):
+        far = (-qb + np.sqrt(np.maximum(disc, 0))) / (2 * qa)
+    shadow = (azi[i] >= 0) & (qa > 0) & (disc >= 0) & (far > 0)
+    # a margin for the device's float arithmetic at the circle's edge
+    edge = (np.abs(disc) < 1e-6) | (np.abs(far) < 1e-6)
+    ok = a["pmt"] >= 0
+    assert (azi >= 0).sum() > 5000 and shadow.sum() > 200
+    bad = ok & shadow & ~edge
+    print("raw hits %d, in the cable's shadow %d (%d near its edge), accepted there %d" % (
+        len(h), shadow.sum(), (shadow & edge).sum(), bad.sum()), flush=True)
+    assert bad.sum() == 0
+    # the others: efficiency x angular sensitivity of the one-PMT sensor
+    rat = np.ctypeslib.as_array(T.rat, shape=(T.nacc * WNUM,)).reshape(T.nacc, WNUM)
+    p = np.zeros(len(h))
+    live = (sens[i] >= 0) & ~shadow
+    for k in np.unique(sens[i][live]):
+        S = T.sensors[int(k)]
+        assert S.def_ == 1
+        m = live & (sens[i] == k)
+        pe = np.minimum(1.0, rat[acc[i[m]], h["z"][m]] * rde[i[m]])
+        x = nx[m] * tilt[i[m], 0] + ny[m] * tilt[i[m], 1] + nz[m] * tilt[i[m], 2]
+        poly = sum(S.s[j] * x ** j for j in range(S.ns))
+        pa = np.clip(poly / S.mas, 0, 1) if S.mas > 0 else None
+        assert pa is not None
+        p[m] = pe * pa
+    want, sd = p[live].sum(), np.sqrt((p * (1 - p))[live].sum())
+    got = (ok & live).sum()
+    print("accepted outside the shadow %d, expected %.1f +- %.1f" % (got, want, sd), flush=True)
+    assert abs(got - want) < 5 * sd + 1
