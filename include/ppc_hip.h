@@ -301,7 +301,9 @@ typedef struct {
   int32_t ns;         /* polynomial coefficients in s (def) */
   int32_t npmt, pmt0; /* PMTs of the type: pmt_dirs[pmt0 .. pmt0+npmt) (not def) */
   float mas;          /* def: maximum of the polynomial (<= 0: cut pos.tilt > s[0]) */
-  float beta, fx, Rr, Rz, cable; /* not def: PMT acceptance curve, its normalisation, size, cable azimuth */
+  float beta, fx, Rr, Rz, cable; /* not def: PMT acceptance curve, its normalisation, size, cable azimuth;
+                                   beta -2: a wavelength-shifting tube of radius Rr and length
+                                   -2 Rz (Rz < 0), read out at its two ends (gpu/ini.cxx:211-243) */
   float s[16];
 } ppc_hip_sensor;
 
