@@ -95,13 +95,14 @@ def gpu_run(args):
     b.num = photons
     for _ in range(args.warmup):
         dev.run([b])
-    ms = []
+    ms, spans = [], []
     steps = hits = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         dev.launch([b])
         s = dev.wait()
         ms.append(s.device_ms)
+        spans.append((s.span_min_ms, s.span_max_ms))
         steps += s.steps
         hits += s.hits
         assert s.photons == photons
@@ -114,6 +115,8 @@ def gpu_run(args):
         "value": photons * args.steps / wall,
         "ms_per_step": wall / args.steps * 1e3,
         "kernel_ms": kern_ms,
+        # in-kernel wall-clock span from the first wave start to the first and the last wave end
+        "in_kernel_ms": [round(sum(a for a, _ in spans) / len(spans), 3), round(sum(b for _, b in spans) / len(spans), 3)],
         "mean_photons": mean,
         "steps_per_photon": steps / (photons * args.steps),
         "hits_per_photon": hits / (photons * args.steps),
@@ -443,6 +446,7 @@ def main():
         "config": {"workload": "86-string cascade: 86x60 DOMs, 170 layers, 1-d tilt, 8%% absorption anisotropy, "
                                "100 TeV EM cascade, %.3g photons per step" % args.photons,
                    "photons_per_step": int(args.photons), "kernel_ms": round(g["kernel_ms"], 3),
+                   "in_kernel_ms": g["in_kernel_ms"],
                    "steps_per_photon": round(g["steps_per_photon"], 2),
                    "hits_per_photon": g["hits_per_photon"], "flop_per_photon": round(g["flop_per_photon"], 1),
                    "events_per_photon": {k: round(v, 4) for k, v in g["events_per_photon"].items()}},

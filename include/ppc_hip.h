@@ -179,6 +179,9 @@ typedef struct {
   uint64_t doms;      /* DOM (ellipsoid) intersection tests */
   uint64_t scatters;  /* scatterings */
   uint64_t hit_photons; /* photons that hit (record_starts; records beyond max_hits are dropped) */
+  /* in-kernel time from the launch's first wave start to its first and to its last wave end,
+   * by the device's wall clock (the reference's in-kernel span, gpu/ppc.cu:267-273,309) */
+  float span_min_ms, span_max_ms;
 } ppc_hip_stats;
 
 /* replay record: MWC state of a lane when it starts emitting photon `photon` */

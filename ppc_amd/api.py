@@ -62,7 +62,7 @@ class Stats(ctypes.Structure):
                 ("stored", ctypes.c_uint64), ("killed", ctypes.c_uint64), ("nonfinite", ctypes.c_uint64),
                 ("device_ms", ctypes.c_float), ("lanes", ctypes.c_int32), ("crossings", ctypes.c_uint64),
                 ("strings", ctypes.c_uint64), ("doms", ctypes.c_uint64), ("scatters", ctypes.c_uint64),
-                ("hit_photons", ctypes.c_uint64)]
+                ("hit_photons", ctypes.c_uint64), ("span_min_ms", ctypes.c_float), ("span_max_ms", ctypes.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

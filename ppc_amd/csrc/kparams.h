@@ -33,7 +33,10 @@ struct Counters {
   unsigned int hidx;   // next free hit slot
   unsigned int nphot;  // photons with hits recorded for the adjoint
   unsigned int waves;  // waves that reached the end of the kernel: all of the launch's
-  unsigned int pad[2];
+  unsigned int pad;
+  // wall clock (wall_clock64 ticks): the launch's earliest wave start and its earliest and
+  // latest wave end; the first two complemented, so that atomicMax on the zeroed block finds them
+  unsigned long long t_start_c, t_end_min_c, t_end_max;
 };
 
 struct KParams {

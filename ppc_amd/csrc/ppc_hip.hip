@@ -110,6 +110,7 @@ hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t stream) {
 struct ppc_hip_ctx {
   int device = 0;
   int cus = 0;
+  int wall_khz = 0;  // rate of the device's wall clock (wall_clock64)
   ppc_hip_config cfg{};
   int lanes = 0;
   hipStream_t stream = nullptr;
@@ -247,6 +248,7 @@ int ppc_hip_open(int32_t device, const ppc_hip_config* cfg, ppc_hip_ctx** out) {
   ppc_hip_ctx* c = new ppc_hip_ctx();
   c->device = device;
   c->cus = prop.multiProcessorCount;
+  if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
   if (cfg) c->cfg = *cfg;
   ppc_hip_config& k = c->cfg;
   if (k.threads <= 0) k.threads = 256;
@@ -985,6 +987,12 @@ int ppc_hip_wait(ppc_hip_ctx* c, ppc_hip_stats* st) {
     st->lanes = c->lanes;
     st->crossings = k.crossings, st->strings = k.strings, st->doms = k.doms, st->scatters = k.scatters;
     st->hit_photons = k.nphot;
+    st->span_min_ms = st->span_max_ms = 0.f;
+    if (k.waves > 0 && c->wall_khz > 0) {
+      const unsigned long long t0 = ~k.t_start_c, e0 = ~k.t_end_min_c, e1 = k.t_end_max;
+      st->span_min_ms = (float)((double)(e0 - t0) / c->wall_khz);
+      st->span_max_ms = (float)((double)(e1 - t0) / c->wall_khz);
+    }
   }
   c->fwd_nphot = c->d_hphot ? std::min<uint32_t>(c->h_ctr->nphot, c->cfg.max_hits) : 0;
   c->fwd_waited = true;

@@ -495,6 +495,8 @@ template <int TILT, bool EXT, bool COUNT, bool ADJ>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void propagate(
     const KParams* __restrict__ Pp) {
   const KParams& P = *Pp;
+  // start of the in-kernel time span, once per workgroup (its waves start together)
+  if (threadIdx.x == 0) atomicMax(&P.ctr->t_start_c, ~wall_clock64());
   extern __shared__ uint4 lds[];
   for (int k = threadIdx.x; k < P.image_words; k += blockDim.x) lds[k] = P.image[k];
   // per lane, after the image: the photon's index, its segment and the MWC state it started
@@ -1113,6 +1115,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kWavesPerS
     for (int q = 0; q < NC; q++)
       if (v[q]) atomicAdd(dst[q], v[q]);
     atomicAdd(&P.ctr->waves, 1u);  // checked on the host: every wave of the launch ran to here
+    const unsigned long long t = wall_clock64();
+    atomicMax(&P.ctr->t_end_min_c, ~t);
+    atomicMax(&P.ctr->t_end_max, t);
   }
 }
 

@@ -72,6 +72,7 @@ struct Backend {
   unsigned hidx = 0;
   unsigned old = 0;
   double device_ms = 0;
+  double span_min_ms = 0, span_max_ms = 0;  // summed in-kernel spans (gpu/ppc.cu:267-273)
   unsigned long long photons = 0, nhits = 0, steps = 0;
 };
 
@@ -183,7 +184,8 @@ void print();
 void start() {}
 
 void stop() {
-  fprintf(stderr, "\nDevice time: %2.1f [ms]\n", be.device_ms);
+  fprintf(stderr, "\nDevice time: %2.1f (in-kernel: %2.1f...%2.1f) [ms]\n", be.device_ms, be.span_min_ms,
+          be.span_max_ms);
   if (be.ctx) ppc_hip_close(be.ctx), be.ctx = nullptr;
 }
 
@@ -310,6 +312,7 @@ static void kernel(unsigned num) {
     ppc_hip_stats st{};
     check(ppc_hip_wait(be.ctx, &st), "wait");
     be.device_ms += st.device_ms;
+    be.span_min_ms += st.span_min_ms, be.span_max_ms += st.span_max_ms;
     be.photons += st.photons, be.nhits += st.hits, be.steps += st.steps;
     if (st.nonfinite > 0) cerr << "Error: TOT was a nan or an inf " << st.nonfinite << " times!" << endl;
     if (st.hits > be.hnum) cerr << "Error: data buffer overflow occurred: " << st.hits << ">" << be.hnum << "!" << endl;
