@@ -25,9 +25,13 @@ def test_empty_launch_then_normal_launch(det):
     print("empty launch", flush=True)
     st = dev.run([])
     assert st.photons == 0 and st.hits == 0 and st.stored == 0 and st.steps == 0
+    assert st.span_min_ms == st.span_max_ms == 0  # no kernel ran
     assert len(dev.hits()) == 0
     st = dev.run([det.flasher_bunch(1, 30, num=1_000_000)])
     assert st.photons == 1_000_000 and st.hits == st.stored > 0
+    # the in-kernel span: first wave end <= last wave end, within the launch's event time
+    print("device %.3f ms, in-kernel %.3f...%.3f ms" % (st.device_ms, st.span_min_ms, st.span_max_ms), flush=True)
+    assert 0 < st.span_min_ms <= st.span_max_ms <= st.device_ms * 1.05 + 0.05
     dev.close()
 
 
