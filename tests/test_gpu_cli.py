@@ -295,6 +295,31 @@ def _run_f2k(lines):
     return photons, np.array(rows), np.array(ev)
 
 
+def test_cli_f2k_echoes_every_line():
+    """`ppc 0 < events.f2k`: every input line, TR or not (headers, event brackets, comments),
+    is echoed in input order, with the HIT lines of a line's photons after it (gpu/f2k.cxx:
+    555-583, 260-267: one segment per line)"""
+    from ppc_amd import _native, configs
+
+    tr = G.cli_f2k_lines(events=1, E=100.0)[0]
+    inp = ["V 2000.1", "HI some header", "EM 1 1 1970 0 0 0", tr.strip(), "EE", "EM 2 1 1970 0 0 0", tr.strip(),
+           "# a comment", "EE", "END"]
+    env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS}
+    env.update({"PPCTABLESDIR": configs.ensure("c2"), "DTMN": "1"})
+    r = subprocess.run([_native.ppc_exe(), "0"], env=env, input="\n".join(inp) + "\n", capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = r.stdout.splitlines()
+    echo = [l for l in out if not l.startswith("HIT ")]
+    assert echo == inp, echo
+    # the hits follow the TR lines that made them: between a TR line and its event's EE
+    pos = [k for k, l in enumerate(out) if not l.startswith("HIT ")]
+    hit_after = [sum(1 for l in out[a + 1:b] if l.startswith("HIT ")) for a, b in zip(pos, pos[1:] + [len(out)])]
+    print("hits after each line: %s" % hit_after, flush=True)
+    assert hit_after[3] > 0 and hit_after[6] > 0
+    assert sum(hit_after) == hit_after[3] + hit_after[6]
+
+
 def test_cli_f2k_cascade_stream_matches_reference_cli():
     """`ppc 0 < events.f2k` on the 86-string ice against the reference command line's HIT lines
     on the same TR lines: 1 TeV EM cascades (tests/golden/cli_f2k_c2.npz, 8 events, 8e7
