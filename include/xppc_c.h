@@ -84,6 +84,11 @@ int xppc_c_addp(float x, float y, float z, float t, float E, float dr, int32_t t
 int xppc_c_eout(void);
 /* number of photon records in hitz; with non-NULL arrays, fills up to cap of them */
 uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* wv, uint64_t cap);
+/* number of keys of hitz; with non-NULL arrays, fills up to cap of them in key order: the key's
+ * (binned) time and direction and the number of photon records under it */
+uint64_t xppc_c_hitz_keys(float* time, float* dir, uint32_t* count, uint64_t cap);
+/* set_res(t, c): time and direction bins of the hitz keys (gpu/f2k.cxx:200-214) */
+int xppc_c_set_res(float t, float c);
 int xppc_c_efin(void);
 /* the acceptance tables of print() in the layout the device post-processing takes
  * (ppc_hip_upload_accept); valid until the next call */

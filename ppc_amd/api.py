@@ -141,6 +141,9 @@ def xppc():
         lib.xppc_c_addp.argtypes = [ctypes.c_float] * 6 + [ctypes.c_int32, ctypes.c_float]
         lib.xppc_c_hitz.restype = ctypes.c_uint64
         lib.xppc_c_hitz.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_uint64]
+        lib.xppc_c_hitz_keys.restype = ctypes.c_uint64
+        lib.xppc_c_hitz_keys.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint64]
+        lib.xppc_c_set_res.argtypes = [ctypes.c_float, ctypes.c_float]
         lib.xppc_c_accept_tables.restype = ctypes.c_void_p
         lib.xppc_c_totals.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -247,6 +250,13 @@ class Detector:
         out = {k: np.zeros(n, dtype=d) for k, d in
                [("str", np.int32), ("dom", np.int32), ("pmt", np.int32), ("t", np.float32), ("wv", np.float32)]}
         self.lib.xppc_c_hitz(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("str", "dom", "pmt", "t", "wv")], n)
+        return out
+
+    def hitz_keys(self):
+        """the keys of the hitz map in order: binned time and direction, photon records under each"""
+        n = int(self.lib.xppc_c_hitz_keys(None, None, None, 0))
+        out = {"time": np.zeros(n, np.float32), "dir": np.zeros(n, np.float32), "count": np.zeros(n, np.uint32)}
+        self.lib.xppc_c_hitz_keys(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("time", "dir", "count")], n)
         return out
 
     def layers(self):

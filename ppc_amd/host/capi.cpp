@@ -202,6 +202,20 @@ uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* 
   return n;
 }
 
+uint64_t xppc_c_hitz_keys(float* time, float* dir, uint32_t* count, uint64_t cap) {
+  uint64_t n = 0;
+  for (const auto& e : hitz) {
+    if (time && n < cap) time[n] = e.first.time, dir[n] = e.first.dir, count[n] = (uint32_t)e.second.size();
+    n++;
+  }
+  return n;
+}
+
+int xppc_c_set_res(float t, float c) {
+  set_res(t, c);
+  return 0;
+}
+
 int xppc_c_efin(void) {
   efin();
   return 0;

@@ -424,6 +424,58 @@ D.stop()
 '''
 
 
+_LIB_RES = r'''
+import sys
+import numpy as np
+from ppc_amd import api
+D = api.Detector(sys.argv[1])
+D.start(0)
+D.lib.xppc_c_set_res(float(sys.argv[3]), float(sys.argv[4]))
+D.lib.xppc_c_flset(1, 30)
+D.lib.xppc_c_sett(0.0, 0.0, 1.0, 1, 0, 0, 0.0)
+D.lib.xppc_c_flone(4000000000)
+D.lib.xppc_c_eout()
+h, k = D.hitz(), D.hitz_keys()
+np.savez(sys.argv[2], **h, **{"key_" + n: v for n, v in k.items()})
+D.stop()
+'''
+
+
+def test_library_hitz_time_and_direction_bins(tmp_path):
+    """set_res(t, c) (gpu/f2k.cxx:200-214, 326-331): the hitz keys carry the hit time in bins of
+    t ns and 1 + cos(photon zenith) in bins of c, each at its bin's centre, and photons that
+    share a key share its entry; the photon records themselves (unbinned time, wavelength) are
+    those of the same deterministic run without bins, photon for photon"""
+    from ppc_amd import configs
+
+    runs = {}
+    for name, trz, crz in (("plain", 0.0, 0.0), ("binned", 5.0, 0.1)):
+        out = str(tmp_path / (name + ".npz"))
+        e = _env()
+        e["PYTHONPATH"] = ROOT
+        print("library flasher, set_res(%g, %g)" % (trz, crz), flush=True)
+        r = subprocess.run([sys.executable, "-c", _LIB_RES, configs.ensure("c1"), out, str(trz), str(crz)], env=e,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        runs[name] = dict(np.load(out))
+    a, b = runs["plain"], runs["binned"]
+    rec = lambda x: np.sort(np.rec.fromarrays([x["str"], x["dom"], x["t"], x["wv"]]))
+    assert len(a["t"]) > 50000 and np.array_equal(rec(a), rec(b))
+    for x in (a, b):
+        assert int(x["key_count"].sum()) == len(x["t"])
+    kt, kd = b["key_time"].astype(np.float64), b["key_dir"].astype(np.float64)
+    assert np.all(np.abs(kt / 5.0 - 0.5 - np.round(kt / 5.0 - 0.5)) < 1e-6)
+    assert np.all(np.abs((kd + 1) / 0.1 - 0.5 - np.round((kd + 1) / 0.1 - 0.5)) < 1e-3)
+    # each photon record lies in the time bin of its key
+    t_of_key = np.repeat(kt, b["key_count"])
+    q = b["t"].astype(np.float64) / 5.0
+    edge = np.abs(q - np.round(q)) < 1e-5  # float rounding may put a time on a bin edge either side
+    assert np.all((np.floor(q) == np.floor(t_of_key / 5.0)) | edge)
+    print("photon records %d; keys %d without bins, %d with 5 ns x 0.1 bins" % (len(a["t"]), len(a["key_time"]),
+                                                                            len(kt)), flush=True)
+    assert len(kt) < 0.9 * len(a["key_time"])
+
+
 def test_library_flasher_hitz(tmp_path):
     from ppc_amd import configs
 
