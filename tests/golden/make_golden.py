@@ -67,6 +67,11 @@ CASES = {
     "c3clst": ("c2", {}, ("clst", dict(dr=5.0, beta=0.77))),
     "c3mopo": ("c2", {}, ("mopo", dict(length=5.0, beta=1.0, tau=40.0, fr=0.5))),
     "c3mopob": ("c2", {}, ("mopo", dict(length=0.0, beta=0.85, tau=0.0, fr=0.7))),
+    # the lab-measured beam of the tilted-LED flasher type (a negative string number: type 2)
+    # and of the second standard candle (string 0, DOM 2: type 4), each with its own beam
+    # constants (gpu/pro.cu:362-372) and LED elevation (gpu/f2k.cxx:640-670)
+    "c1t2lab": ("c1", {"WFLA": "405", "FWID": "1090", "FLDR": "30"}, ("flasher_env", -1, 30)),
+    "c1sclab": ("c1", {"WFLA": "405", "FWID": "1090"}, ("flasher_env", 0, 2)),
 }
 # single-bunch source for the replay of the muon case: a 1 m segment of the 1 TeV muon
 C3_SEGMENT = dict(x=6.0, y=-4.0, z=-30.0, E=1000.0, dr=1.0, n=(0.6, -0.48, -0.64))
@@ -217,8 +222,9 @@ def _stats_worker(args):
 
 
 # launches of 524288 photons per worker (8 workers) where the default of 8 is too few: the 1 km
-# track spreads its light over the whole string, so its per-DOM comparison needs more photons
-STATS_BATCHES = {"c3km": 32}
+# track spreads its light over the whole string, so its per-DOM comparison needs more photons;
+# the standard candle sits 55 m outside the strings and few of its photons hit
+STATS_BATCHES = {"c3km": 32, "c1sclab": 64}
 STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 
