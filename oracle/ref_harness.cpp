@@ -155,6 +155,20 @@ double ref_mopo(float x, float y, float zz, float t, float nx, float ny, float n
   return light * q.eff / ovr;
 }
 
+// The bunch addp() builds for light of the flset()-configured flasher (type < 0: -1 a linear
+// cone, -2 a 2-d gaussian, -3 isotropic, all around the direction n that sett() set, -4 the
+// flasher's own beam; gpu/f2k.cxx:484-504), the components llh fits (llh/llh.cxx:1484-1492),
+// taken from the reference's own global bunch after the call; the photons it queues are
+// dropped.  Returns the Poisson mean of its count.
+double ref_led(float x, float y, float zz, float t, float E, int type, float nx, float ny, float nz,
+               orc_bunch* out) {
+  p.n.x = nx, p.n.y = ny, p.n.z = nz;
+  addp(x, y, zz, t, E, 0, type);
+  from_photon(p, *out);
+  pn = 0, pk = 0;
+  return yield(E, 0, type) / ovr;
+}
+
 // the flasher bunch flset() builds (gpu/f2k.cxx:630-759)
 void ref_flasher(int str, int dom, orc_bunch* out) {
   flset(str, dom);

@@ -393,6 +393,17 @@ def ref_mopo(x, y, z, t, n, light, length, beta, tau, fr):
     return b, mean
 
 
+def ref_led(x, y, z, t, E, ptype, n):
+    """the reference's addp() bunch for light of the configured flasher (ptype < 0) with
+    direction n, and its mean photon count"""
+    lib = reference()
+    lib.ref_led.restype = ctypes.c_double
+    lib.ref_led.argtypes = [ctypes.c_float] * 5 + [ctypes.c_int] + [ctypes.c_float] * 3 + [ctypes.POINTER(Bunch)]
+    b = Bunch()
+    mean = lib.ref_led(x, y, z, t, E, ptype, n[0], n[1], n[2], ctypes.byref(b))
+    return b, mean
+
+
 def ref_flasher(string, dom):
     lib = reference()
     lib.ref_flasher.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Bunch)]

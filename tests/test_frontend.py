@@ -1,7 +1,8 @@
 """The product's front-end (ppc_amd/host/frontend.cpp) against the reference's
 own: the bunches addp(), addp_clst(), addp_mopo() and flset() build
 (gpu/f2k.cxx:17-167, 484-528, 630-759) for every particle type, for the IceTray
-entry points and for flashers under the beam-shaping environment variables,
+entry points, for flashers under the beam-shaping environment variables and for
+the light of a configured flasher added through addp (as llh's flasher fits do),
 compared field by field with the reference's CPU build."""
 import math
 import os
@@ -64,6 +65,23 @@ for s, d in [(1, 30), (2, 10), (0, 1), (0, 2), (-3, 5)]:
     if U.bunch_fields(rb) != U.bunch_fields(pb):
         bad += 1
         print("MISMATCH flasher", s, d, U.bunch_fields(rb), U.bunch_fields(pb))
+# light of the configured flasher through addp (type < 0), as llh's flasher fits add it
+# (llh/llh.cxx:1484-1540): last, since the reference's addp draws its Poisson count
+for s, d in [(1, 30), (-3, 5)]:
+    U.ref_flasher(s, d)
+    D.flasher_bunch(s, d)
+    for ptype in [-1, -2, -3, -4]:
+        for k in range(5):
+            x, y, z, t = rng.uniform(-300, 300, 4)
+            E = float(np.exp(rng.uniform(0, 12)))
+            n = rng.normal(size=3); n /= np.linalg.norm(n)
+            rb, rmean = U.ref_led(x, y, z, t, E, ptype, n)
+            pb, pmean = D.particle_bunch(x, y, z, t, E, 0.0, ptype, n, num=0)
+            ob = U.to_orc_bunch(pb)
+            rb.q = ob.q = 0
+            if U.bunch_fields(rb) != U.bunch_fields(ob) or abs(rmean - pmean) > 1e-6 * max(1.0, abs(rmean)):
+                bad += 1
+                print("MISMATCH led", s, d, ptype, U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
 print("BAD", bad)
 '''
 
