@@ -169,6 +169,10 @@ double ref_led(float x, float y, float zz, float t, float E, int type, float nx,
   return yield(E, 0, type) / ovr;
 }
 
+// flshift(m, n, r) (gpu/f2k.cxx:590-628): n and r turned by the rotation that takes the z axis
+// onto the DOM axis m (n with a NaN z: the LED elevation of the last flset() first)
+void ref_flshift(float* m, float* n, float* r) { flshift(m, n, r); }
+
 // the flasher bunch flset() builds (gpu/f2k.cxx:630-759)
 void ref_flasher(int str, int dom, orc_bunch* out) {
   flset(str, dom);

@@ -211,6 +211,12 @@ uint64_t xppc_c_hitz_keys(float* time, float* dir, uint32_t* count, uint64_t cap
   return n;
 }
 
+int xppc_c_flshift(const float* m, float* n, float* r) {
+  float mm[3] = {m[0], m[1], m[2]};
+  flshift(mm, n, r);
+  return 0;
+}
+
 int xppc_c_set_res(float t, float c) {
   set_res(t, c);
   return 0;

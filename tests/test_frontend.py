@@ -82,6 +82,38 @@ for s, d in [(1, 30), (-3, 5)]:
             if U.bunch_fields(rb) != U.bunch_fields(ob) or abs(rmean - pmean) > 1e-6 * max(1.0, abs(rmean)):
                 bad += 1
                 print("MISMATCH led", s, d, ptype, U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
+# flshift: the beam directions llh turns onto a tilted DOM's axis (llh/llh.cxx:1526-1533)
+import ctypes
+ref = U.reference()
+ref.ref_flshift.argtypes = [ctypes.POINTER(ctypes.c_float)] * 3
+D.lib.xppc_c_flshift.argtypes = [ctypes.POINTER(ctypes.c_float)] * 3
+F3 = ctypes.c_float * 3
+def shift(fn, m, n, r):
+    mm, nn, rr = F3(*m), F3(*n), F3(*r)
+    fn(mm, nn, rr)
+    return np.array(list(nn) + list(rr), dtype=np.float64)
+up = None
+for k in range(40):
+    m = rng.normal(size=3); m /= np.linalg.norm(m)
+    if k < 4:
+        m = np.array([0.0, 0.0, 1.0]) if k < 2 else np.array([0.0, 0.0, -1.0])
+    n = rng.normal(size=3); n /= np.linalg.norm(n)
+    r = rng.uniform(-1, 1, 3)
+    # the two builds round differently (the reference's with fast-math)
+    a, b = shift(ref.ref_flshift, m, n, r), shift(D.lib.xppc_c_flshift, m, n, r)
+    if not np.allclose(a, b, rtol=0, atol=1e-6):
+        bad += 1
+        print("MISMATCH flshift", m, n, a, b)
+    # a NaN z takes the LED elevation of the last flset() (the tilted LED of flasher -3 5:
+    # 48.1 deg + FZCR, or ELEV; gpu/f2k.cxx:667-679) first; the reference's own check is
+    # compiled away by fast-math here, so the product is held to the definition instead
+    nn = n.copy(); nn[2] = np.nan
+    import os
+    e = np.radians(float(os.environ["ELEV"]) if "ELEV" in os.environ else 48.1 + float(os.environ.get("FZCR", 0)))
+    want = shift(D.lib.xppc_c_flshift, m, [n[0] * np.cos(e), n[1] * np.cos(e), np.sin(e)], r)
+    if not np.allclose(shift(D.lib.xppc_c_flshift, m, nn, r), want, rtol=0, atol=1e-6):
+        bad += 1
+        print("MISMATCH flshift elevation", m, n)
 print("BAD", bad)
 '''
 
