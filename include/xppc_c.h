@@ -71,6 +71,9 @@ float xppc_c_zshift(float x, float y, float z, float* dz);
 
 /* the host random-number state (seed of the MWC streams, gpu/ini.cxx:554) */
 uint32_t xppc_c_seed(void);
+/* the host generator's next uniform in (0, 1] and gaussian (xppc::xrnd, xppc::grnd; gpu/f2k.cxx:1-10) */
+float xppc_c_xrnd(void);
+float xppc_c_grnd(void);
 
 /* the library-mode pipeline of the xppc interface (what llh drives, llh/ppc.h):
  * choose + ini, flasher set-up, track segments, particles, flush, and the hitz

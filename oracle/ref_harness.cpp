@@ -169,6 +169,11 @@ double ref_led(float x, float y, float zz, float t, float E, int type, float nx,
   return yield(E, 0, type) / ovr;
 }
 
+// the host generator (gpu/f2k.cxx:1-10) and its seed
+float ref_xrnd() { return xrnd(); }
+float ref_grnd() { return grnd(); }
+unsigned int ref_seed() { return sv; }
+
 // flshift(m, n, r) (gpu/f2k.cxx:590-628): n and r turned by the rotation that takes the z axis
 // onto the DOM axis m (n with a NaN z: the LED elevation of the last flset() first)
 void ref_flshift(float* m, float* n, float* r) { flshift(m, n, r); }

@@ -150,6 +150,8 @@ float xppc_c_zshift(float x, float y, float z, float* dz) {
 }
 
 uint32_t xppc_c_seed(void) { return sv; }
+float xppc_c_xrnd(void) { return xrnd(); }
+float xppc_c_grnd(void) { return grnd(); }
 
 int xppc_c_start(int32_t device) {
   if (!g_init) return -1;

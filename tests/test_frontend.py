@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 
 _CODE = r'''
-import sys, math
+import ctypes, sys, math
 import numpy as np
 sys.path.insert(0, sys.argv[2])
 import oracle_util as U
@@ -27,6 +27,20 @@ T = U.ref_tables(sys.argv[1])
 D = api.Detector(sys.argv[1])
 bad = 0
 rng = np.random.default_rng(1)
+# the host generator llh draws from (xppc::xrnd, xppc::grnd): the same seed and sequence
+# after initialisation (both draw equally below, up to the LED light at the end)
+ref = U.reference()
+ref.ref_xrnd.restype = ref.ref_grnd.restype = ctypes.c_float
+ref.ref_seed.restype = ctypes.c_uint
+D.lib.xppc_c_xrnd.restype = D.lib.xppc_c_grnd.restype = ctypes.c_float
+seq = lambda fx, fg: [fx() for _ in range(2000)] + [fg() for _ in range(2000)]
+if ref.ref_seed() != D.lib.xppc_c_seed():
+    bad += 1
+    print("MISMATCH seed", ref.ref_seed(), D.lib.xppc_c_seed())
+a, b = np.array(seq(ref.ref_xrnd, ref.ref_grnd)), np.array(seq(D.lib.xppc_c_xrnd, D.lib.xppc_c_grnd))
+if not (np.array_equal(a[:2000], b[:2000]) and np.allclose(a[2000:], b[2000:], rtol=1e-5, atol=1e-6)):
+    bad += 1
+    print("MISMATCH host generator", a[:4], b[:4], np.abs(a - b).max())
 for ptype in [0, 1, 10, 11, 12, 13, 14, 101, 102, 103, 104, 105, 106, 107]:
     for k in range(20):
         x, y, z, t = rng.uniform(-300, 300, 4)
@@ -83,8 +97,6 @@ for s, d in [(1, 30), (-3, 5)]:
                 bad += 1
                 print("MISMATCH led", s, d, ptype, U.bunch_fields(rb), U.bunch_fields(ob), rmean, pmean)
 # flshift: the beam directions llh turns onto a tilted DOM's axis (llh/llh.cxx:1526-1533)
-import ctypes
-ref = U.reference()
 ref.ref_flshift.argtypes = [ctypes.POINTER(ctypes.c_float)] * 3
 D.lib.xppc_c_flshift.argtypes = [ctypes.POINTER(ctypes.c_float)] * 3
 F3 = ctypes.c_float * 3
