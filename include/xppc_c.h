@@ -90,6 +90,9 @@ uint64_t xppc_c_hitz(int32_t* str, int32_t* dom, int32_t* pmt, float* t, float* 
 /* number of keys of hitz; with non-NULL arrays, fills up to cap of them in key order: the key's
  * (binned) time and direction and the number of photon records under it */
 uint64_t xppc_c_hitz_keys(float* time, float* dir, uint32_t* count, uint64_t cap);
+/* number of keys of hitz; with non-NULL arrays, fills up to cap of them in key order: the
+ * segment each key belongs to (the mcid sett() gave it: frame, id, sub-id, t0) */
+uint64_t xppc_c_hitz_tracks(int32_t* frame, int32_t* id, uint64_t* sub, double* t0, uint64_t cap);
 /* flshift(m, n, r): turn n (and r unless NULL) by the rotation that takes the z axis onto the
  * DOM axis m; n with a NaN z first gets the LED elevation of the last flset (gpu/f2k.cxx:590-628) */
 int xppc_c_flshift(const float* m, float* n, float* r);

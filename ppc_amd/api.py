@@ -144,6 +144,8 @@ def xppc():
         lib.xppc_c_hitz_keys.restype = ctypes.c_uint64
         lib.xppc_c_hitz_keys.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint64]
         lib.xppc_c_set_res.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.xppc_c_hitz_tracks.restype = ctypes.c_uint64
+        lib.xppc_c_hitz_tracks.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint64]
         lib.xppc_c_accept_tables.restype = ctypes.c_void_p
         lib.xppc_c_totals.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -257,6 +259,14 @@ class Detector:
         n = int(self.lib.xppc_c_hitz_keys(None, None, None, 0))
         out = {"time": np.zeros(n, np.float32), "dir": np.zeros(n, np.float32), "count": np.zeros(n, np.uint32)}
         self.lib.xppc_c_hitz_keys(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("time", "dir", "count")], n)
+        return out
+
+    def hitz_tracks(self):
+        """the segment (mcid: frame, id, sub-id, t0) of each hitz key, in key order"""
+        n = int(self.lib.xppc_c_hitz_tracks(None, None, None, None, 0))
+        out = {"frame": np.zeros(n, np.int32), "id": np.zeros(n, np.int32), "sub": np.zeros(n, np.uint64),
+               "t0": np.zeros(n, np.float64)}
+        self.lib.xppc_c_hitz_tracks(*[out[k].ctypes.data_as(ctypes.c_void_p) for k in ("frame", "id", "sub", "t0")], n)
         return out
 
     def layers(self):

@@ -213,6 +213,18 @@ uint64_t xppc_c_hitz_keys(float* time, float* dir, uint32_t* count, uint64_t cap
   return n;
 }
 
+uint64_t xppc_c_hitz_tracks(int32_t* frame, int32_t* id, uint64_t* sub, double* t0, uint64_t cap) {
+  uint64_t n = 0;
+  for (const auto& e : hitz) {
+    if (frame && n < cap) {
+      const mcid& m = e.first.track;
+      frame[n] = m.frame, id[n] = m.first, sub[n] = m.second, t0[n] = m.t0;
+    }
+    n++;
+  }
+  return n;
+}
+
 int xppc_c_flshift(const float* m, float* n, float* r) {
   float mm[3] = {m[0], m[1], m[2]};
   flshift(mm, n, r);

@@ -476,6 +476,63 @@ def test_library_hitz_time_and_direction_bins(tmp_path):
     assert len(kt) < 0.9 * len(a["key_time"])
 
 
+_LIB_EVENTS = r'''
+import sys
+import numpy as np
+from ppc_amd import api
+D = api.Detector(sys.argv[1])
+oms = D.array("oms", np.float32).reshape(-1, 5)[:, 2:5]
+key = D.array("str", np.int32) * 100 + D.array("dom", np.int32)
+pos = {s: oms[int(np.flatnonzero(key == s * 100 + 30)[0])] + np.float32(5.0) for s in (36, 80)}
+D.start(0)
+# three segments, as llh's sett() calls give them: two 10 TeV cascades next to strings 36 and 80,
+# and a segment without light
+D.lib.xppc_c_sett(0.0, 0.0, -1.0, 1, 10, 0, 0.0)
+D.lib.xppc_c_addp(*[float(v) for v in pos[36]], 0.0, 1e4, 0.0, 11, 1.0)
+D.lib.xppc_c_sett(0.0, 0.0, -1.0, 2, 20, 1, 5.0)
+D.lib.xppc_c_addp(*[float(v) for v in pos[80]], 0.0, 1e4, 0.0, 11, 1.0)
+D.lib.xppc_c_sett(0.0, 0.0, -1.0, 3, 30, 2, 0.0)
+D.lib.xppc_c_eout()
+h, k, tr = D.hitz(), D.hitz_keys(), D.hitz_tracks()
+np.savez(sys.argv[2], **h, **{"key_" + n: v for n, v in k.items()}, **{"tr_" + n: v for n, v in tr.items()},
+         p36=pos[36], p80=pos[80], oms=oms, omkey=key)
+D.stop()
+'''
+
+
+def test_library_segments_keep_their_hits(tmp_path):
+    """Library mode with three segments (sett with distinct ids, frames and t0, gpu/f2k.cxx:548-552):
+    a cascade next to string 36, one next to string 80, and a segment without light.  Each hitz
+    key carries the segment its photons came from, keys are ordered by segment (frame first,
+    mcid's order), no key belongs to the empty segment, and each segment's hits lie around
+    its own cascade"""
+    from ppc_amd import configs
+
+    out = str(tmp_path / "events.npz")
+    e = _env()
+    e.pop("WFLA", None), e.pop("FWID", None)
+    e["PYTHONPATH"] = ROOT
+    r = subprocess.run([sys.executable, "-c", _LIB_EVENTS, configs.ensure("c2"), out], env=e, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    x = np.load(out)
+    segs = list(zip(x["tr_frame"].tolist(), x["tr_id"].tolist(), x["tr_sub"].tolist(), x["tr_t0"].tolist()))
+    assert set(segs) == {(0, 1, 10, 0.0), (1, 2, 20, 5.0)}
+    order = [0 if s[0] == 0 else 1 for s in segs]
+    assert order == sorted(order)  # frame 0's keys first
+    seg = np.repeat(np.array(order), x["key_count"])
+    idx = {int(k): i for i, k in enumerate(x["omkey"])}
+    where = x["oms"][[idx[int(s) * 100 + int(d)] for s, d in zip(x["str"], x["dom"])]]
+    d36 = np.linalg.norm(where - x["p36"], axis=1)
+    d80 = np.linalg.norm(where - x["p80"], axis=1)
+    n0, n1 = int((seg == 0).sum()), int((seg == 1).sum())
+    print("hits: segment 1 %d, segment 2 %d; median distance to own / other cascade %.0f / %.0f m, %.0f / %.0f m"
+          % (n0, n1, np.median(d36[seg == 0]), np.median(d80[seg == 0]), np.median(d80[seg == 1]),
+             np.median(d36[seg == 1])), flush=True)
+    assert n0 > 1000 and n1 > 1000
+    assert np.mean(d36[seg == 0] < d80[seg == 0]) > 0.99 and np.mean(d80[seg == 1] < d36[seg == 1]) > 0.99
+
+
 def test_library_flasher_hitz(tmp_path):
     from ppc_amd import configs
 
