@@ -169,6 +169,21 @@ double ref_led(float x, float y, float zz, float t, float E, int type, float nx,
   return yield(E, 0, type) / ovr;
 }
 
+// per DOM, what print() reads for its acceptance (gpu/f2k.cxx:269-324): module type,
+// wavelength type, relative efficiency, high voltage, cable azimuth and axis, then the
+// maximum and the class efficiency of its (module, wavelength) class, whose acceptance per
+// wavelength bin goes to rat (up to cap entries).  Returns that table's length.
+int ref_dom_accept(int i, float* out, float* rat, int cap) {
+  name& n = q.names[i];
+  out[0] = n.omt, out[1] = n.type, out[2] = n.rde, out[3] = n.hv, out[4] = n.azi;
+  out[5] = n.tilt[0], out[6] = n.tilt[1], out[7] = n.tilt[2];
+  irde& r = irdes[std::make_pair(n.omt, n.type)];
+  out[8] = r.rmax, out[9] = r.rde;
+  const int m = (int)r.rat.size();
+  for (int k = 0; k < m && k < cap; k++) rat[k] = r.rat[k];
+  return m;
+}
+
 // the host generator (gpu/f2k.cxx:1-10) and its seed
 float ref_xrnd() { return xrnd(); }
 float ref_grnd() { return grnd(); }

@@ -309,6 +309,27 @@ def write_c2x(path, seed=3, nrnd=131072):
     return path
 
 
+def write_c2xb(path, nrnd=131072):
+    """The full-feature 86-string ice (write_c2x) with the two remaining per-DOM and per-string
+    inputs of ini.cxx: DOM orientations (cx.dat: the axis of every DOM on strings 36-44, tilted
+    by up to 10 deg) and fixed positions of the hole-ice columns (str-f2k: strings 36-44 moved
+    by up to 0.5 m from their DOMs' average)."""
+    write_c2x(path, nrnd=nrnd)
+    rng = np.random.default_rng(11)
+    with open(os.path.join(path, "cx.dat"), "w") as f:
+        for st in range(36, 45):
+            for d in range(1, NDOM + 1):
+                th, ph = math.radians(rng.uniform(0.0, 10.0)), rng.uniform(0.0, 2 * math.pi)
+                f.write("%d %d %.6f %.6f %.6f 0.5\n" % (st, d, math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph),
+                                                       -math.cos(th)))
+    strings = _string_positions(86)
+    with open(os.path.join(path, "str-f2k"), "w") as f:
+        for st in range(36, 45):
+            x, y = strings[st - 1]
+            f.write("%d %.3f %.3f\n" % (st, x + rng.uniform(-0.5, 0.5), y + rng.uniform(-0.5, 0.5)))
+    return path
+
+
 def c3_muon_event(seed=4, length=1200.0, step=1.0, E=1000.0):
     """The muon workload on the 86-string ice: a 1 TeV muon on a straight 1.2 km
     track through the array centre at a zenith in [0, 80] deg, as f2k track
@@ -481,7 +502,7 @@ def write_c1wom(path, nrnd=131072):
 
 
 _WRITERS = {"c1": write_c1, "c2": write_c2, "c2x": write_c2x, "c2t9": write_c2t9, "cbig": write_cbig,
-            "c1ng": write_c1ng, "c1wom": write_c1wom}
+            "c1ng": write_c1ng, "c1wom": write_c1wom, "c2xb": write_c2xb}
 
 
 def ensure(name, nrnd=131072):

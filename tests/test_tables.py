@@ -23,7 +23,7 @@ ROOT = os.path.dirname(HERE)
 REFICE = "/root/reference/ice"
 
 ENVS = {"c1": {"WFLA": "405", "FWID": "10"}, "c2": {}, "c2x": {}, "c1ng": {"WFLA": "405", "FWID": "10"},
-        "c1wom": {"WFLA": "405", "FWID": "10"}}
+        "c1wom": {"WFLA": "405", "FWID": "10"}, "c2xb": {}}
 
 
 def _child(code, *args, env=None):
@@ -66,7 +66,7 @@ for k in U.TABLE_FIELDS:
 
 
 @pytest.mark.ref
-@pytest.mark.parametrize("cfg", ["c1", "c2", "c2x", "c1ng", "c1wom"])
+@pytest.mark.parametrize("cfg", ["c1", "c2", "c2x", "c1ng", "c1wom", "c2xb"])
 def test_tables_match_reference(cfg):
     if not U.ref_available():
         pytest.skip("oracle/_ref not built (no reference tree)")
