@@ -273,11 +273,12 @@ def _permutation_chi2(E1, E2, nperm=2000, seed=1, min_exp=10):
     return obs, len(groups) - 1, float((1 + (null >= obs).sum()) / (nperm + 1))
 
 
-def _run_f2k(lines):
+def _run_f2k(lines, extra=None):
     from ppc_amd import _native, configs
 
-    env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS}
+    env = {k: v for k, v in os.environ.items() if k not in G.ENV_KEYS and k != "PPC_ACCEPT"}
     env.update({"PPCTABLESDIR": configs.ensure("c2"), "DTMN": "1"})
+    env.update(extra or {})
     r = subprocess.run([_native.ppc_exe(), "0"], env=env, input="".join(lines), capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -320,13 +321,16 @@ def test_cli_f2k_echoes_every_line():
     assert sum(hit_after) == hit_after[3] + hit_after[6]
 
 
-def test_cli_f2k_cascade_stream_matches_reference_cli():
+@pytest.mark.parametrize("accept", ["host", "device"])
+def test_cli_f2k_cascade_stream_matches_reference_cli(accept):
     """`ppc 0 < events.f2k` on the 86-string ice against the reference command line's HIT lines
     on the same TR lines: 1 TeV EM cascades (tests/golden/cli_f2k_c2.npz, 8 events, 8e7
-    photons); 8e8 photons here, in one process"""
+    photons); 8e8 photons here, in one process; with the acceptance on the host and on the
+    device (the full emission spectrum: hits in every wavelength bin, each with its own
+    acceptance and wavelength range)"""
     lines = G.cli_f2k_lines(events=80)
-    print("f2k stream: %d cascade TR lines on c2" % len(lines), flush=True)
-    photons, arr, _ = _run_f2k(lines)
+    print("f2k stream: %d cascade TR lines on c2, acceptance on the %s" % (len(lines), accept), flush=True)
+    photons, arr, _ = _run_f2k(lines, {"PPC_ACCEPT": "device"} if accept == "device" else None)
     assert photons > 5e8
     _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_f2k_c2.npz")
 
