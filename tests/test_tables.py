@@ -215,6 +215,27 @@ def test_cli_zshift(cfg):
     np.testing.assert_allclose(got, ref, rtol=1e-5, atol=2e-4)
 
 
+@pytest.mark.ref
+@pytest.mark.parametrize("ice", ["spice_bfr-v2", "spice_ftp-v3m"])
+def test_cli_table_dump_real_ice(ice):
+    """`ppc -` and `ppc - x y` on the reference's own ice directories against the reference
+    program's output (oracle/_ref, run here)"""
+    path = os.path.join(REFICE, ice)
+    if not U.ref_available() or not os.path.isdir(path):
+        pytest.skip("reference ice directories not present")
+    for args in (["-"], ["-", "310.5", "-205.25"]):
+        e = dict(os.environ)
+        for k in ("WFLA", "FWID", "VTHK"):
+            e.pop(k, None)
+        e.update({"PPCTABLESDIR": path, "DTMN": "1"})
+        ref = subprocess.run([U.REF_EXE] + args, env=e, capture_output=True, text=True, timeout=120)
+        assert ref.returncode == 0, ref.stderr[-2000:]
+        got = _rows(_ppc(args, {"PPCTABLESDIR": path}))
+        want = _rows(ref.stdout)
+        assert got.shape == want.shape
+        np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-6)
+
+
 @pytest.mark.parametrize("cfg", ["c2", "c2x"])
 def test_cli_table_dump_at_position(cfg):
     """`ppc - x y`: the same table with the depth shift taken at a horizontal position"""
