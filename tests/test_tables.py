@@ -218,8 +218,8 @@ def test_cli_zshift(cfg):
 @pytest.mark.ref
 @pytest.mark.parametrize("ice", ["spice_bfr-v2", "spice_ftp-v3m"])
 def test_cli_table_dump_real_ice(ice):
-    """`ppc -` and `ppc - x y` on the reference's own ice directories against the reference
-    program's output (oracle/_ref, run here)"""
+    """`ppc -`, `ppc - x y` and `ppc =` on the reference's own ice directories (1-d and 2-d
+    tilt) against the reference program's output (oracle/_ref, run here)"""
     path = os.path.join(REFICE, ice)
     if not U.ref_available() or not os.path.isdir(path):
         pytest.skip("reference ice directories not present")
@@ -234,6 +234,17 @@ def test_cli_table_dump_real_ice(ice):
         want = _rows(ref.stdout)
         assert got.shape == want.shape
         np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-6)
+    # `ppc =`: the depth shift of the tilted layers at points across the detector (VTHK=1)
+    pts = np.random.default_rng(9).uniform([-700, -700, -600], [700, 700, 600], (300, 3))
+    inp = "\n".join("%.3f %.3f %.3f" % tuple(p) for p in pts) + "\n"
+    e = dict(os.environ)
+    for k in ("WFLA", "FWID"):
+        e.pop(k, None)
+    e.update({"PPCTABLESDIR": path, "DTMN": "1", "VTHK": "1"})
+    ref = subprocess.run([U.REF_EXE, "="], env=e, input=inp, capture_output=True, text=True, timeout=120)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    got = _rows(_ppc(["="], {"PPCTABLESDIR": path, "VTHK": "1"}, stdin=inp))
+    np.testing.assert_allclose(got, _rows(ref.stdout), rtol=1e-5, atol=2e-4)
 
 
 @pytest.mark.parametrize("cfg", ["c2", "c2x"])
