@@ -56,8 +56,10 @@ CASES = {
     "c2vthk": ("c2", {"VTHK": "1"}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
     # a cascade in a detector near the reference's capacity (200 strings, 14280 DOMs)
     "cbig": ("cbig", {}, ("cascade", 8.0, 5.0, -51.93, -0.6, -0.3)),
-    # the full-feature ice with hole ice only for photons later than 300 ns and DOMs 3x oversized
+    # the full-feature ice with hole ice only for photons later than 300 ns and DOMs 3x oversized,
+    # and with hole ice only for photons earlier than 300 ns (a negative HIFL, gpu/pro.cu:655)
     "c2xho": ("c2x", {"HIFL": "300", "OVSZ": "3"}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
+    "c2xhe": ("c2x", {"HIFL": "-300"}, ("cascade", 5.0, -6.0, 120.0, -0.5, 0.6)),
     # one whole 1 km down-going 1 TeV muon (the amu TR line of the command line's f2k stream test)
     "c3km": ("c2", {}, ("muon_km",)),
     # the IceTray entry points (gpu/f2k.cxx:507-528): a 5 m segment of a particle at beta = 0.77,
@@ -223,8 +225,9 @@ def _stats_worker(args):
 
 # launches of 524288 photons per worker (8 workers) where the default of 8 is too few: the 1 km
 # track spreads its light over the whole string, so its per-DOM comparison needs more photons;
-# the standard candle sits 55 m outside the strings and few of its photons hit
-STATS_BATCHES = {"c3km": 32, "c1sclab": 64}
+# the standard candle sits 55 m outside the strings and few of its photons hit; the
+# full-feature ice's rate is compared at 0.1 %
+STATS_BATCHES = {"c3km": 32, "c1sclab": 64, "c2x": 32}
 STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 

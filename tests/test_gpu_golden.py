@@ -101,9 +101,9 @@ def _two_sample_chi2(a, b, min_exp=10):
     return chi2, dof, float(stats.chi2.sf(chi2, dof))
 
 
-SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 100_000_000, "c3": 10,  # photons; c3: event repeats
+SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 400_000_000, "c3": 10,  # photons; c3: event repeats
            "c2t9": 100_000_000, "c1iso": 200_000_000, "c1lab": 200_000_000, "c1cone": 200_000_000,
-           "c2vthk": 100_000_000, "cbig": 100_000_000, "c2xho": 100_000_000, "c3km": 1_000_000_000,
+           "c2vthk": 100_000_000, "cbig": 100_000_000, "c2xho": 100_000_000, "c2xhe": 100_000_000, "c3km": 1_000_000_000,
            "c3clst": 200_000_000, "c3mopo": 200_000_000, "c3mopob": 200_000_000, "c1t2lab": 200_000_000,
            "c1sclab": 1_000_000_000}
 
