@@ -228,6 +228,7 @@ def _stats_worker(args):
 # the standard candle sits 55 m outside the strings and few of its photons hit; the
 # full-feature ice's rate is compared at 0.1 %
 STATS_BATCHES = {"c3km": 32, "c1sclab": 64, "c2x": 32}
+STATS_PROCS = {"c2x": 16}  # 16 workers: 2.7e8 reference photons
 STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 
@@ -471,10 +472,10 @@ if __name__ == "__main__":
             make_replay(c)
     if "stats" in what:
         for c in CASES:
-            make_stats(c, batches=STATS_BATCHES.get(c, 8))
+            make_stats(c, procs=STATS_PROCS.get(c, 8), batches=STATS_BATCHES.get(c, 8))
     for c in CASES:
         if "stats_" + c in what:
-            make_stats(c, batches=STATS_BATCHES.get(c, 8))
+            make_stats(c, procs=STATS_PROCS.get(c, 8), batches=STATS_BATCHES.get(c, 8))
         if "replay_" + c in what:
             make_replay(c)
     if "cli" in what:
