@@ -227,8 +227,8 @@ def _stats_worker(args):
 # track spreads its light over the whole string, so its per-DOM comparison needs more photons;
 # the standard candle sits 55 m outside the strings and few of its photons hit; the
 # full-feature ice's rate is compared at 0.1 %
-STATS_BATCHES = {"c3km": 32, "c1sclab": 64, "c2x": 32}
-STATS_PROCS = {"c2x": 16}  # 16 workers: 2.7e8 reference photons
+STATS_BATCHES = {"c3km": 32, "c1sclab": 64, "c2x": 32, "c2xhe": 32}
+STATS_PROCS = {"c2x": 16, "c2xhe": 16}  # 16 workers: 2.7e8 reference photons
 STATS_SAMPLE = 40000  # hits kept for the time and direction distributions
 
 

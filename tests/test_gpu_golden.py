@@ -103,7 +103,7 @@ def _two_sample_chi2(a, b, min_exp=10):
 
 SOURCES = {"c1": 200_000_000, "c2": 100_000_000, "c2x": 400_000_000, "c3": 10,  # photons; c3: event repeats
            "c2t9": 100_000_000, "c1iso": 200_000_000, "c1lab": 200_000_000, "c1cone": 200_000_000,
-           "c2vthk": 100_000_000, "cbig": 100_000_000, "c2xho": 100_000_000, "c2xhe": 100_000_000, "c3km": 1_000_000_000,
+           "c2vthk": 100_000_000, "cbig": 100_000_000, "c2xho": 100_000_000, "c2xhe": 400_000_000, "c3km": 1_000_000_000,
            "c3clst": 200_000_000, "c3mopo": 200_000_000, "c3mopob": 200_000_000, "c1t2lab": 200_000_000,
            "c1sclab": 1_000_000_000}
 
