@@ -119,6 +119,17 @@ def test_cli_small_launches_and_hit_quota():
     assert all(int(h) <= cap for _, h in launches) and 0 < nh <= cap * len(launches)
 
 
+def test_cli_without_arguments_lists_devices():
+    """`ppc` alone lists the devices and prints the usage (gpu/ppc.cu:404-425, 525-530)"""
+    from ppc_amd import _native
+
+    r = subprocess.run([_native.ppc_exe()], env=_env(), capture_output=True, text=True, timeout=120)
+    print(r.stderr.strip(), flush=True)
+    assert r.returncode == 0 and r.stdout == ""
+    assert re.search(r"Found 1 devices", r.stderr) and re.search(r"^0: .* CUs\(256\) G\(\d+\)$", r.stderr, re.M)
+    assert "[device] (f2k muons)" in r.stderr and "[str] [om] [num] [device] (flasher)" in r.stderr
+
+
 @pytest.mark.parametrize("var", ["NPHO", "NPHO_0"])
 def test_cli_npho_zero_drops_the_device(var):
     """NPHO=0 (or NPHO_<device>=0) takes the device out of use, and with no device left the
