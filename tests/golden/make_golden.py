@@ -337,12 +337,13 @@ def make_mwc():
               open(os.path.join(HERE, "mwc_rnd.json"), "w"), indent=1)
 
 
-def make_cli(n=int(1.6e9), cfg="c1", flasher=(1, 30)):
+def make_cli(n=int(1.6e9), cfg="c1", flasher=(1, 30), procs=8):
     """accepted HIT lines of the reference command line on a flasher: the C1 flasher, or a
-    flasher of the 86-string ice with every optional feature and cable positions (c2x)"""
+    flasher of the 86-string ice with every optional feature and cable positions (c2x), also
+    with tilted DOM axes and fixed hole-ice columns (c2xb)"""
     env = _env({"WFLA": "405", "FWID": "10", "PPCTABLESDIR": configs.ensure(cfg)})
     procs = [subprocess.Popen([U.REF_EXE, str(flasher[0]), str(flasher[1]), "%d" % (n // 8), str(k)], env=env,
-                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for k in range(8)]
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for k in range(procs)]
     # one thread per process, so that the processes run side by side and none blocks on a full pipe
     with ThreadPoolExecutor(len(procs)) as ex:
         res = list(ex.map(lambda p: p.communicate(), procs))
@@ -484,6 +485,8 @@ if __name__ == "__main__":
         make_cli_nextgen()
     if "cli" in what or "cli_c2x" in what:
         make_cli(n=int(1.6e8), cfg="c2x", flasher=(36, 30))
+    if "cli" in what or "cli_c2xb" in what:
+        make_cli(n=int(1.6e8), cfg="c2xb", flasher=(36, 30), procs=16)
     if "cli" in what or "cli_wom" in what:
         make_cli_nextgen(cfg="c1wom")
     if "cli" in what or "cli_f2k" in what:

@@ -158,20 +158,23 @@ def test_cli_flasher_matches_reference_cli():
     _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3])
 
 
+@pytest.mark.parametrize("cfg", ["c2x", "c2xb"])
 @pytest.mark.parametrize("accept", ["host", "device"])
-def test_cli_flasher_full_feature_ice_matches_reference_cli(accept):
+def test_cli_flasher_full_feature_ice_matches_reference_cli(accept, cfg):
     """`ppc 36 30 N 0` on the 86-string ice with every optional feature (2-d tilt,
     birefringence, hole ice, both anisotropies) and cable positions (dx.dat, whose shadow
     print() applies, gpu/f2k.cxx:301-322) against the reference command line's HIT lines
-    (tests/golden/cli_c2x.npz, 1.1e8 photons); with the acceptance on the host and on the
-    device: HIT rate, per-DOM counts, arrival times and wavelengths"""
+    (tests/golden/cli_<cfg>.npz, 1.1e8 photons); c2xb adds tilted DOM axes (cx.dat: the
+    flasher's beam, gpu/f2k.cxx:730-740, and the angular acceptance of the hits) and fixed
+    hole-ice columns (str-f2k); with the acceptance on the host and on the device: HIT rate,
+    per-DOM counts, arrival times and wavelengths"""
     from ppc_amd import _native, configs
 
     env = _env()
-    env["PPCTABLESDIR"] = configs.ensure("c2x")
+    env["PPCTABLESDIR"] = configs.ensure(cfg)
     if accept == "device":
         env["PPC_ACCEPT"] = "device"
-    print("c2x flasher 36-30, acceptance on the %s" % accept, flush=True)
+    print("%s flasher 36-30, acceptance on the %s" % (cfg, accept), flush=True)
     r = subprocess.run([_native.ppc_exe(), "36", "30", "3.2e8", "0"], env=env, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -179,7 +182,7 @@ def test_cli_flasher_full_feature_ice_matches_reference_cli(accept):
     photons = sum(int(m) for m in re.findall(r"photons: (\d+)", r.stderr))
     arr = np.array([[float(v) for v in l.split()[1:5]] for l in r.stdout.splitlines() if l.startswith("HIT ")])
     assert photons > 2e8
-    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_c2x.npz")
+    _compare(photons, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], fixture="cli_%s.npz" % cfg)
 
 
 @pytest.mark.parametrize("cfg", ["c1ng", "c1wom"])
